@@ -1,0 +1,239 @@
+/*
+ * sphx.h - C ABI of the MI355X-native SPH-EXA single-rank hydro hot path.
+ *
+ * Every entry point takes plain device pointers and sizes, runs asynchronously
+ * on the stream held by its context, and returns an int status (SPHX_OK = 0).
+ * A failing call leaves a message retrievable with sphx_last_error(ctx).
+ * Nothing here calls exit(); nothing uses process-global device state, so two
+ * contexts on one device are independent (calls that overlap on ONE context race).
+ *
+ * Citations name the reference C++ interface each entry point replaces
+ * (paths relative to the SPH-EXA source tree).
+ */
+#ifndef SPHX_H
+#define SPHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum
+{
+    SPHX_OK          = 0,
+    SPHX_ERR_HIP     = 1, /* HIP runtime error (message has the HIP error string)          */
+    SPHX_ERR_ARG     = 2, /* invalid argument (null pointer, bad size, bad precision)      */
+    SPHX_ERR_STACK   = 3, /* traversal queue overflow (domain/.../find_neighbors.cuh:252)  */
+    SPHX_ERR_NGMAX   = 4, /* ng0 > ngmax (sph/include/sph/find_neighbors.hpp:49)           */
+    SPHX_ERR_STATE   = 5, /* SPH loop called without matching neighbor lists in context   */
+    SPHX_ERR_NOMEM   = 6, /* device allocation failed                                      */
+    SPHX_ERR_NODEV   = 7  /* no HIP device                                                 */
+};
+
+/* cstone::BoundaryType, domain/include/cstone/sfc/box.hpp:80-85 */
+enum
+{
+    SPHX_BC_OPEN     = 0,
+    SPHX_BC_PERIODIC = 1,
+    SPHX_BC_FIXED    = 2
+};
+
+/* element type of the hydro fields (everything except x, y, z and keys) */
+enum
+{
+    SPHX_FP64  = 0, /* double fields, FP64 arithmetic everywhere                         */
+    SPHX_MIXED = 1  /* float fields; FP64 positions and pair offsets, FP32 weights and
+                       Kahan-compensated FP32 neighbor sums                               */
+};
+
+/* cstone::Box<double>, domain/include/cstone/sfc/box.hpp:94-174 (lengths and inverse
+ * lengths are derived inside the library exactly as the reference constructor does) */
+typedef struct sphx_box
+{
+    double  xmin, xmax, ymin, ymax, zmin, zmax;
+    int32_t bx, by, bz; /* SPHX_BC_* */
+    int32_t reserved;
+} sphx_box;
+
+/* Internal octree + neighbor-search view; mirrors cstone::OctreeView
+ * (domain/include/cstone/tree/octree.hpp:224-245) and cstone::OctreeNsView (:259-281).
+ * All pointers are device pointers, allocated by the caller with the sizes given
+ * by sphx_octree_sizes(). */
+typedef struct sphx_octree
+{
+    int32_t   numLeafNodes;
+    int32_t   numInternalNodes; /* (numLeafNodes - 1) / 7                                */
+    int32_t   numNodes;         /* numLeafNodes + numInternalNodes                        */
+    int32_t   maxDepth;         /* deepest level with nodes (filled by sphx_build_octree) */
+    uint64_t* prefixes;         /* [numNodes] Warren-Salmon placeholder-bit keys          */
+    int32_t*  childOffsets;     /* [numNodes + 1], 0 = leaf                               */
+    int32_t*  parents;          /* [max(1, (numNodes - 1) / 8)]                           */
+    int32_t*  levelRange;       /* [23] first node index per level, device               */
+    int32_t*  internalToLeaf;   /* [numNodes]                                             */
+    int32_t*  leafToInternal;   /* [numNodes]                                             */
+    uint64_t* leaves;           /* [numLeafNodes + 1] cornerstone leaf keys               */
+    uint32_t* layout;           /* [numLeafNodes + 1] first particle of each leaf         */
+    double*   centers;          /* [3 * numNodes] node centres, xyz interleaved           */
+    double*   sizes;            /* [3 * numNodes] node half-sizes, xyz interleaved        */
+} sphx_octree;
+
+/* sph::GroupView, domain/include/cstone/traversal/groups.hpp:28-34.  Target groups
+ * hold at most 64 particles (one wave64).  groupStart == NULL selects fixed groups of
+ * 64 consecutive particles covering [firstBody, lastBody). */
+typedef struct sphx_groups
+{
+    uint32_t        firstBody, lastBody, numGroups;
+    const uint32_t* groupStart; /* device [numGroups] */
+    const uint32_t* groupEnd;   /* device [numGroups] */
+} sphx_groups;
+
+/* The particle fields consumed by the hot path; mirrors the field contract of
+ * sphexa::ParticlesData (sph/include/sph/particles_data.hpp:218-246, 85-143).
+ * x, y, z are always double.  Every void* field is double* when precision ==
+ * SPHX_FP64 and float* when precision == SPHX_MIXED.  wh is the kernel table of
+ * SPHX_TABLE_SIZE entries in the same element type. */
+typedef struct sphx_particles
+{
+    uint64_t      n;
+    int32_t       precision;
+    uint32_t      ng0, ngmax;
+    float         Kcour;
+    double        K, gamma;
+    const double *x, *y, *z;
+    void *        h, *m, *vx, *vy, *vz, *u;
+    void *        rho, *p, *c;
+    void *        c11, *c12, *c13, *c22, *c23, *c33;
+    void *        ax, *ay, *az, *du;
+    uint32_t*     nc;
+    const void*   wh;
+    double        minDtCourant; /* output of sphx_compute_momentum_energy */
+} sphx_particles;
+
+#define SPHX_TABLE_SIZE 20000 /* sph::lt::kTableSize, sph/include/sph/table_lookup.hpp:10 */
+#define SPHX_MAX_LEVEL 21     /* maxTreeLevel<uint64_t>, domain/include/cstone/tree/definitions.h:55 */
+
+typedef struct sphx_ctx sphx_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+/* Creates a context on `device` that launches on `stream` (a hipStream_t, NULL =
+ * the device's null stream).  The context owns all scratch memory. */
+int         sphx_ctx_create(int device, void* stream, sphx_ctx** out);
+int         sphx_ctx_destroy(sphx_ctx* ctx);
+int         sphx_ctx_set_stream(sphx_ctx* ctx, void* stream);
+int         sphx_ctx_synchronize(sphx_ctx* ctx);
+const char* sphx_last_error(const sphx_ctx* ctx);
+const char* sphx_version(void);
+
+/* ---- kernel tables ------------------------------------------------------- */
+/* Host-side table and normalisation: replaces ParticlesData::createTables
+ * (sph/include/sph/particles_data.hpp:377-387): K = kernel_3D_k(sinc^n, 2)
+ * (sph_kernel_tables.hpp:78-85) and wh = tabulateFunction (:88-102) in double and
+ * in float.  Any output pointer may be NULL. */
+int sphx_kernel_tables(double sincIndex, double* wh64, float* wh32, double* K);
+
+/* ---- SFC keys ------------------------------------------------------------ */
+/* cstone::computeSfcKeys<true> (domain/include/cstone/sfc/sfc_gpu.h:26-31,
+ * CPU semantics sfc/sfc.hpp:267-275): 64-bit Hilbert keys; entries already equal
+ * to removeKey (2^63) are left untouched. */
+int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const double* z, uint64_t* keys,
+                          uint64_t n, const sphx_box* box);
+
+/* ---- sort / permute ------------------------------------------------------ */
+/* cstone::sortByKey<true> (domain/include/cstone/primitives/primitives_acc.hpp:115-121,
+ * primitives_gpu.cu:308-339): stable ascending LSD radix sort of (key, value)
+ * pairs, in place.  Scratch comes from the context. */
+int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n);
+/* cstone::sequence<true> (primitives_acc.hpp:108-113): values[i] = first + i */
+int sphx_sequence(sphx_ctx* ctx, uint32_t* values, uint64_t n, uint32_t first);
+/* cstone::gather (domain/include/cstone/primitives/gather.hpp:90-98): out[i] = in[order[i]] */
+int sphx_gather_f64(sphx_ctx* ctx, const uint32_t* order, const double* in, double* out, uint64_t n);
+int sphx_gather_f32(sphx_ctx* ctx, const uint32_t* order, const float* in, float* out, uint64_t n);
+int sphx_gather_u32(sphx_ctx* ctx, const uint32_t* order, const uint32_t* in, uint32_t* out, uint64_t n);
+
+/* ---- cornerstone octree --------------------------------------------------- */
+/* cstone::computeOctree (domain/include/cstone/tree/csarray.hpp:430-440; GPU loop
+ * updateOctreeGpu, tree/update_gpu.cuh:41-64): the converged cornerstone leaf array
+ * of sorted `keys` for `bucketSize`.  Results are written into context-owned device
+ * memory, valid until the next sphx_compute_octree on this context:
+ * *leaves -> [*numLeaves + 1] keys, *counts -> [*numLeaves] particle counts
+ * (capped at maxCount), *layout -> [*numLeaves + 1] exclusive scan of the counts.
+ * Synchronises the stream once (to return numLeaves). */
+int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_t bucketSize, uint32_t maxCount,
+                        int32_t* numLeaves, uint64_t** leaves, uint32_t** counts, uint32_t** layout);
+
+/* Array sizes of an internal octree with numLeafNodes leaves (OctreeData::resize,
+ * tree/octree.hpp:291-306). */
+int sphx_octree_sizes(int32_t numLeafNodes, int32_t* numInternalNodes, int32_t* numNodes, int32_t* numParents);
+
+/* cstone::buildOctreeGpu / updateInternalTree (tree/octree_gpu.h:34,
+ * tree/octree_gpu.cu:138-188, tree/octree.hpp:168-196,361-367): link the internal
+ * octree of `leaves` into the caller-allocated arrays of `tree` (prefixes,
+ * childOffsets, parents, levelRange, internalToLeaf, leafToInternal; numLeafNodes
+ * must be set; the other counts are filled in). */
+int sphx_build_octree(sphx_ctx* ctx, const uint64_t* leaves, sphx_octree* tree);
+
+/* cstone::nodeFpCenters / computeGeoCentersGpu (focus/source_center.hpp:146-158,
+ * focus/source_center_gpu.cu:222-235): geometric centre and half-size of every node. */
+int sphx_node_centers(sphx_ctx* ctx, const uint64_t* prefixes, int32_t numNodes, const sphx_box* box,
+                      double* centers, double* sizes);
+
+/* ---- target groups -------------------------------------------------------- */
+/* sph::computeGroups -> cstone::computeGroupSplits (sph/include/sph/groups.hpp:41-74,
+ * traversal/groups_gpu.cu:108-137): SFC-consecutive groups of at most 64
+ * particles, split where consecutive particles are farther apart than
+ * min(tolFactor * cbrt(smallest leaf volume in the group), h).  Writes group
+ * boundaries into context memory; *groups is filled with device pointers valid
+ * until the next call. */
+int sphx_compute_groups(sphx_ctx* ctx, uint32_t first, uint32_t last, const double* x, const double* y,
+                        const double* z, const void* h, int32_t precision, const sphx_octree* tree,
+                        const sphx_box* box, float tolFactor, sphx_groups* groups);
+
+/* ---- neighbor search ------------------------------------------------------ */
+/* cstone::findNeighbors (domain/include/cstone/findneighbors.hpp:149-170): for
+ * i in [first, last), the neighbors j != i with d^2(i,j) < 4 h_i^2 (minimum image
+ * for periodic dimensions when the search sphere leaves the box).  nbrs
+ * (optional, may be NULL) uses the reference CPU layout nbrs[(i - first) * ngmax + k],
+ * in ascending j, truncated at ngmax; nc[i - first] is the full count. */
+int sphx_find_neighbors(sphx_ctx* ctx, const double* x, const double* y, const double* z, const double* h,
+                        uint32_t first, uint32_t last, const sphx_box* box, const sphx_octree* tree,
+                        uint32_t ngmax, uint32_t* nbrs, uint32_t* nc);
+
+/* sph::findNeighborsSfc -> findNeighborsSph (sph/include/sph/find_neighbors.hpp:10-53):
+ * neighbor search with up to 10 smoothing-length updates (updateH, kernels.hpp:27-33)
+ * per particle outside [ng0/4, ngmax + 1]; d->nc[i] counts self.  The lists are kept
+ * in the context for the SPH loops below, grouped by `groups`. */
+int sphx_find_neighbors_sph(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box,
+                            const sphx_octree* tree);
+
+/* Copy the context's lists (from sphx_find_neighbors_sph) into the reference CPU
+ * layout nbrs[(i - firstBody) * ngmax + k]. */
+int sphx_export_neighbors(sphx_ctx* ctx, uint32_t* nbrs);
+
+/* ---- std-SPH particle loops ------------------------------------------------ */
+/* sph::computeDensity (sph/include/sph/hydro_std/density.hpp:41-60, xmassJLoop
+ * hydro_ve/xmass_kern.hpp:50-79): rho_i = K/h_i^3 (m_i + sum_j m_j W(r_ij/h_i)). */
+int sphx_compute_density(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box);
+
+/* sph::computeEOS_HydroStd, ideal gas (hydro_std/eos.hpp:52-78,144-150; sph/eos.hpp:34-44):
+ * p = rho (gamma - 1) u, c = sqrt(gamma (gamma - 1) u) for i in [first, last). */
+int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d);
+
+/* sph::computeIAD (hydro_std/iad.hpp:41-79, IADJLoopSTD hydro_std/iad_kern.hpp:12-78):
+ * the inverse of the symmetric tau tensor -> c11, c12, c13, c22, c23, c33. */
+int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box);
+
+/* sph::computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:41-101,
+ * momentumAndEnergyJLoop hydro_std/momentum_energy_kern.hpp:12-134): ax, ay, az,
+ * du and d->minDtCourant (Courant time step, kernels.hpp:10-17).  Synchronises
+ * the stream once to return minDtCourant. */
+int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
+                                 const sphx_box* box);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHX_H */

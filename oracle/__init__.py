@@ -1,0 +1,276 @@
+"""CPU checker for the SPH hot path (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+package.  It binds two shared libraries with ctypes:
+
+* ``liboracle.so``: the C++ restatement of the reference CPU path
+  (oracle/sphx_oracle.cpp), always built by oracle/Makefile;
+* ``_ref/libsphexa_ref.so``: the reference's own CPU templates compiled from its
+  headers (oracle/ref_driver.cpp); present only where the reference sources were
+  available at build time.
+
+``Cpu(lib)`` wraps either library behind one numpy interface, so a test can run
+the same call on both and compare.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsphexa_ref.so")
+
+TABLE_SIZE = 20000
+BC_OPEN, BC_PERIODIC, BC_FIXED = 0, 1, 2
+
+
+class Box(C.Structure):
+    """sphx_box (include/sphx.h)."""
+
+    _fields_ = [("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+                ("zmin", C.c_double), ("zmax", C.c_double), ("bx", C.c_int32), ("by", C.c_int32),
+                ("bz", C.c_int32), ("reserved", C.c_int32)]
+
+    @classmethod
+    def cube(cls, lo, hi, bc=BC_OPEN):
+        return cls(lo, hi, lo, hi, lo, hi, bc, bc, bc, 0)
+
+    @classmethod
+    def make(cls, lo3, hi3, bc3):
+        return cls(lo3[0], hi3[0], lo3[1], hi3[1], lo3[2], hi3[2], bc3[0], bc3[1], bc3[2], 0)
+
+    def lengths(self):
+        return np.array([self.xmax - self.xmin, self.ymax - self.ymin, self.zmax - self.zmin])
+
+
+def build(quiet=True):
+    """Compile liboracle.so (and _ref when the reference sources are present)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+_D = C.c_double
+_U = C.c_uint32
+_I = C.c_int
+_V = C.c_void_p
+_U64 = C.c_uint64
+
+
+class Cpu:
+    """numpy front end for liboracle.so (prefix 'orc') or libsphexa_ref.so (prefix 'ref')."""
+
+    def __init__(self, kind="oracle"):
+        path = ORACLE_SO if kind == "oracle" else REF_SO
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.kind = kind
+        self.pre = "orc" if kind == "oracle" else "ref"
+        self.lib = C.CDLL(path)
+        f = self._f
+        f("kernel_tables", None, [_D, _V, _V, _V])
+        f("sphynx_k", _D, [_D])
+        f("ihilbert", _U64, [_U, _U, _U])
+        f("decode_hilbert", None, [_U64, _V])
+        f("compute_sfc_keys", None, [_V, _V, _V, _V, _U64, _V])
+        f("sort_by_key", None, [_V, _V, _U64])
+        f("compute_octree", _I, [_V, _U64, _U, _U, _V, _V, _I])
+        f("build_octree", None, [_V, _I, _V, _V, _V, _V, _V, _V])
+        f("node_centers", None, [_V, _I, _V, _V, _V])
+        if kind == "oracle":
+            f("find_neighbors", None, [_V, _V, _V, _V, _U, _U, _V, _V, _V, _V, _V, _V, _V, _U, _V, _V])
+            f("find_neighbors_sph", None, [_V, _V, _V, _V, _U, _U, _V, _V, _V, _V, _V, _V, _V, _U, _U, _V, _V])
+            f("density", None, [_U, _U, _V, _V, _U, _V, _V, _V, _V, _V, _V, _D, _V, _V])
+            f("eos", None, [_U, _U, _V, _V, _D, _V, _V])
+            f("iad", None, [_U, _U, _V, _V, _U] + [_V] * 7 + [_D, _V] + [_V] * 6)
+            f("momentum_energy", _D, [_U, _U, _V, _V, _U] + [_V] * 18 + [_D, C.c_float, _V] + [_V] * 4)
+            f("iad_one", None, [_U, _V, _U] + [_V] * 7 + [_D, _V, _V])
+            f("momentum_energy_one", None, [_U, _V, _U] + [_V] * 18 + [_D, _V, _V])
+            f("rebalance_decision", _I, [_V, _V, _I, _U, _V])
+            f("update_octree", _I, [_V, _U64, _U, _V, _V, _V, _I])
+            f("update_h", _D, [_U, _U, _D])
+            f("simpson_sin", _D, [_D, _D, _U64])
+            f("lookup", _D, [_V, _D])
+        else:
+            f("find_neighbors", None, [_V, _V, _V, _V, _U, _U, _V, _I, _V, _V, _V, _V, _V, _V, _V, _V, _U, _V, _V])
+            f("find_neighbors_sph", None,
+              [_V, _V, _V, _V, _U, _U, _V, _I, _V, _V, _V, _V, _V, _V, _V, _V, _U, _U, _V, _V])
+            f("density", None, [_U64, _U, _U, _V, _V, _U, _V, _V, _V, _V, _V, _V, _D, _V, _V])
+            f("eos", None, [_U64, _U, _U, _V, _V, _D, _V, _V])
+            f("iad", None, [_U64, _U, _U, _V, _V, _U] + [_V] * 7 + [_D, _V] + [_V] * 6)
+            f("momentum_energy", _D, [_U64, _U, _U, _V, _V, _U] + [_V] * 18 + [_D, C.c_float, _V] + [_V] * 4)
+            f("hydro_step", _D, [_U64] + [_V] * 9 + [_V, _U, _U, _U, _D, _D, C.c_float] + [_V] * 5)
+        f("xmass_one", _D, [_U, _V, _U, _V, _V, _V, _V, _V, _V, _D, _V])
+
+    def _f(self, name, res, args):
+        fn = getattr(self.lib, f"{self.pre}_{name}")
+        fn.restype = res
+        fn.argtypes = args
+        setattr(self, "_" + name, fn)
+
+    # ---------------------------------------------------------------- tables
+    def kernel_tables(self, sinc_index=6.0):
+        wh64 = np.zeros(TABLE_SIZE, np.float64)
+        wh32 = np.zeros(TABLE_SIZE, np.float32)
+        K = np.zeros(1)
+        self._kernel_tables(sinc_index, _p(wh64), _p(wh32), _p(K))
+        return wh64, wh32, float(K[0])
+
+    def sphynx_k(self, n):
+        return self._sphynx_k(n)
+
+    # ---------------------------------------------------------------- keys
+    def ihilbert(self, x, y, z):
+        return int(self._ihilbert(x, y, z))
+
+    def decode_hilbert(self, key):
+        out = np.zeros(3, np.uint32)
+        self._decode_hilbert(key, _p(out))
+        return tuple(int(v) for v in out)
+
+    def sfc_keys(self, x, y, z, box, keys=None):
+        n = len(x)
+        keys = np.zeros(n, np.uint64) if keys is None else keys.copy()
+        self._compute_sfc_keys(_p(x), _p(y), _p(z), _p(keys), n, C.byref(box))
+        return keys
+
+    def sort_by_key(self, keys, vals):
+        k = np.ascontiguousarray(keys, np.uint64).copy()
+        v = np.ascontiguousarray(vals, np.uint32).copy()
+        self._sort_by_key(_p(k), _p(v), len(k))
+        return k, v
+
+    # ---------------------------------------------------------------- tree
+    def compute_octree(self, keys, bucket, max_count=0xFFFFFFFF):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        cap = max(16, len(keys) + 8)
+        while True:
+            leaves = np.zeros(cap, np.uint64)
+            counts = np.zeros(cap, np.uint32)
+            L = self._compute_octree(_p(keys), len(keys), bucket, max_count, _p(leaves), _p(counts), cap)
+            if L > 0:
+                return leaves[:L + 1].copy(), counts[:L].copy()
+            cap = -L + 8
+
+    def build_octree(self, leaves):
+        L = len(leaves) - 1
+        ni = (L - 1) // 7
+        nn = L + ni
+        o = dict(prefixes=np.zeros(nn, np.uint64), childOffsets=np.zeros(nn + 1, np.int32),
+                 parents=np.zeros(max(1, (nn - 1) // 8), np.int32), levelRange=np.zeros(23, np.int32),
+                 internalToLeaf=np.zeros(nn, np.int32), leafToInternal=np.zeros(nn, np.int32))
+        self._build_octree(_p(np.ascontiguousarray(leaves, np.uint64)), L, _p(o["prefixes"]), _p(o["childOffsets"]),
+                           _p(o["parents"]), _p(o["levelRange"]), _p(o["internalToLeaf"]), _p(o["leafToInternal"]))
+        o["childOffsets"] = o["childOffsets"][:nn]
+        o.update(numLeafNodes=L, numInternalNodes=ni, numNodes=nn)
+        return o
+
+    def node_centers(self, prefixes, box):
+        nn = len(prefixes)
+        c = np.zeros((nn, 3))
+        s = np.zeros((nn, 3))
+        self._node_centers(_p(np.ascontiguousarray(prefixes, np.uint64)), nn, C.byref(box), _p(c), _p(s))
+        return c, s
+
+    def tree(self, keys, box, bucket=64):
+        """The single-rank harness (findneighbors.cpp:59-83): leaves, internal tree, layout, centres."""
+        leaves, counts = self.compute_octree(keys, bucket)
+        o = self.build_octree(leaves)
+        layout = np.zeros(len(leaves), np.uint32)
+        layout[1:] = np.cumsum(counts, dtype=np.uint64).astype(np.uint32)
+        c, s = self.node_centers(o["prefixes"], box)
+        o.update(leaves=leaves, counts=counts, layout=layout, centers=c, sizes=s)
+        return o
+
+    # ---------------------------------------------------------------- neighbors
+    def find_neighbors(self, x, y, z, h, box, t, ngmax, first=0, last=None, lists=True):
+        last = len(x) if last is None else last
+        n = last - first
+        nc = np.zeros(n, np.uint32)
+        nb = np.zeros(n * ngmax, np.uint32) if lists else None
+        if self.kind == "oracle":
+            self._find_neighbors(_p(x), _p(y), _p(z), _p(h), first, last, C.byref(box), _p(t["childOffsets"]),
+                                 _p(t["parents"]), _p(t["internalToLeaf"]), _p(t["layout"]), _p(t["centers"]),
+                                 _p(t["sizes"]), ngmax, _p(nb), _p(nc))
+        else:
+            if nb is None:
+                nb = np.zeros(n * ngmax, np.uint32)
+            self._find_neighbors(_p(x), _p(y), _p(z), _p(h), first, last, C.byref(box), t["numLeafNodes"],
+                                 _p(t["prefixes"]), _p(t["childOffsets"]), _p(t["parents"]), _p(t["internalToLeaf"]),
+                                 _p(t["levelRange"]), _p(t["layout"]), _p(t["centers"]), _p(t["sizes"]), ngmax,
+                                 _p(nb), _p(nc))
+        return (nb.reshape(n, ngmax) if nb is not None else None), nc
+
+    def find_neighbors_sph(self, x, y, z, h, box, t, ng0, ngmax, first=0, last=None):
+        """Returns (lists[n, ngmax], nc with self, h updated copy)."""
+        last = len(x) if last is None else last
+        n = last - first
+        h = np.array(h, np.float64, copy=True)
+        nc = np.zeros(n, np.uint32)
+        nb = np.zeros(n * ngmax, np.uint32)
+        if self.kind == "oracle":
+            self._find_neighbors_sph(_p(x), _p(y), _p(z), _p(h), first, last, C.byref(box), _p(t["childOffsets"]),
+                                     _p(t["parents"]), _p(t["internalToLeaf"]), _p(t["layout"]), _p(t["centers"]),
+                                     _p(t["sizes"]), ng0, ngmax, _p(nb), _p(nc))
+        else:
+            self._find_neighbors_sph(_p(x), _p(y), _p(z), _p(h), first, last, C.byref(box), t["numLeafNodes"],
+                                     _p(t["prefixes"]), _p(t["childOffsets"]), _p(t["parents"]),
+                                     _p(t["internalToLeaf"]), _p(t["levelRange"]), _p(t["layout"]), _p(t["centers"]),
+                                     _p(t["sizes"]), ng0, ngmax, _p(nb), _p(nc))
+        return nb.reshape(n, ngmax), nc, h
+
+    # ---------------------------------------------------------------- SPH loops (FP64)
+    def _n(self, *a):
+        return [] if self.kind == "oracle" else [a[0]]
+
+    def density(self, nb, nc, ngmax, x, y, z, h, m, wh, K, box, first=0, last=None):
+        n = len(x)
+        last = n if last is None else last
+        rho = np.zeros(n)
+        self._density(*self._n(n), first, last, _p(nb), _p(nc), ngmax, _p(x), _p(y), _p(z), _p(h), _p(m), _p(wh), K,
+                      C.byref(box), _p(rho))
+        return rho
+
+    def eos(self, u, rho, gamma, first=0, last=None):
+        n = len(u)
+        last = n if last is None else last
+        p = np.zeros(n)
+        c = np.zeros(n)
+        self._eos(*self._n(n), first, last, _p(u), _p(rho), gamma, _p(p), _p(c))
+        return p, c
+
+    def iad(self, nb, nc, ngmax, x, y, z, h, m, rho, wh, K, box, first=0, last=None):
+        n = len(x)
+        last = n if last is None else last
+        cs = [np.zeros(n) for _ in range(6)]
+        self._iad(*self._n(n), first, last, _p(nb), _p(nc), ngmax, _p(x), _p(y), _p(z), _p(h), _p(m), _p(rho),
+                  _p(wh), K, C.byref(box), *[_p(a) for a in cs])
+        return cs
+
+    def momentum_energy(self, nb, nc, ngmax, x, y, z, vx, vy, vz, h, m, rho, p, c, cs, wh, K, Kcour, box, first=0,
+                        last=None):
+        n = len(x)
+        last = n if last is None else last
+        out = [np.zeros(n) for _ in range(4)]
+        dt = self._momentum_energy(*self._n(n), first, last, _p(nb), _p(nc), ngmax, _p(x), _p(y), _p(z), _p(vx),
+                                   _p(vy), _p(vz), _p(h), _p(m), _p(rho), _p(p), _p(c), *[_p(a) for a in cs], _p(wh),
+                                   K, Kcour, C.byref(box), *[_p(a) for a in out])
+        return out[0], out[1], out[2], out[3], dt
+
+    def xmass_one(self, i, nbrs, x, y, z, h, m, wh, K, box):
+        nb = np.ascontiguousarray(nbrs, np.uint32)
+        return self._xmass_one(i, _p(nb), len(nb), _p(x), _p(y), _p(z), _p(h), _p(m), _p(wh), K, C.byref(box))
+
+
+def available(kind):
+    return os.path.exists(ORACLE_SO if kind == "oracle" else REF_SO)

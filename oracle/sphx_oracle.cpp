@@ -1,0 +1,964 @@
+/*
+ * sphx_oracle.cpp - CPU restatement of SPH-EXA's single-rank std-hydro path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke() entry
+ * point and bench.py's cpu_baseline leg load it; the product library
+ * (sphexa_amd/csrc) never links or calls it.
+ *
+ * Every function restates the reference algorithm it names (file:line relative to
+ * the SPH-EXA source tree) with the same operation order, in double precision,
+ * parallelised with OpenMP like the reference CPU path.  Built with
+ * -ffp-contract=off so that every product and sum rounds separately (the
+ * reference Release build would otherwise contract to FMA per compiler whim).
+ *
+ * How each part is pinned (see tests/test_oracle.py):
+ *   keys        decode(encode) = id, curve continuity, first-level octant map,
+ *               hilbertIBox known answers (domain/test/unit/sfc/hilbert.cpp:83-296),
+ *               and equality with the compiled reference (oracle/_ref).
+ *   sort        brute-force std::sort on (key, index) pairs + stability check.
+ *   octree      csarray known answers (domain/test/unit/tree/csarray.cpp:102-206),
+ *               cornerstone invariants, equality with oracle/_ref.
+ *   link        octree.cpp known shapes (4x4x4 -> 73 nodes, irregular 15 leaves),
+ *               connectivity invariants, equality with oracle/_ref.
+ *   neighbors   brute-force all pairs (domain/test/unit/neighbors/all_to_all.hpp)
+ *               and equality with oracle/_ref.
+ *   density     ve.cpp XMass known answer (rho0 = 34.515038498081417),
+ *               brute-force sums, equality with oracle/_ref.
+ *   IAD, ME     std.cpp known answers (sph/test/std.cpp:98-143), oracle/_ref.
+ *   tables, K   Simpson known answers (sph/test/table_creation.cpp:17-40),
+ *               closed-form integral.
+ *   EOS         closed form.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../include/sphx.h"
+
+namespace
+{
+
+using Key = uint64_t;
+
+constexpr unsigned kMaxLevel   = 21;                 // tree/definitions.h:55
+constexpr Key      kRemoveKey  = Key(1) << 63;       // tree/definitions.h:70-75
+constexpr Key      kRootRange  = Key(1) << 63;       // nodeRange(0), sfc/common.hpp:108-115
+constexpr int      kTableSize  = SPHX_TABLE_SIZE;    // sph/table_lookup.hpp:10
+
+// ---------------------------------------------------------------------------
+// box (sfc/box.hpp:94-174)
+struct Box
+{
+    double lo[3], hi[3], len[3], ilen[3];
+    int    bc[3];
+};
+
+Box makeBox(const sphx_box* b)
+{
+    Box r;
+    r.lo[0] = b->xmin, r.hi[0] = b->xmax;
+    r.lo[1] = b->ymin, r.hi[1] = b->ymax;
+    r.lo[2] = b->zmin, r.hi[2] = b->zmax;
+    r.bc[0] = b->bx, r.bc[1] = b->by, r.bc[2] = b->bz;
+    for (int d = 0; d < 3; ++d)
+    {
+        r.len[d]  = r.hi[d] - r.lo[d];
+        r.ilen[d] = 1.0 / (r.hi[d] - r.lo[d]);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// key bit algebra (sfc/common.hpp)
+inline unsigned clz64(Key k) { return k == 0 ? 64u : unsigned(__builtin_clzll(k)); }
+
+inline Key nodeRange(unsigned level) { return Key(1) << (3u * (kMaxLevel - level)); } // common.hpp:108-115
+
+inline unsigned treeLevel(Key range) { return (clz64(range - 1) - 1) / 3; } // common.hpp:156-161
+
+inline int commonPrefix(Key a, Key b) { return int(clz64(a ^ b)) - 1; } // common.hpp:144-148
+
+inline Key encodePlaceholderBit(Key code, int prefixLength) // common.hpp:172-180
+{
+    int nShifts = 3 * int(kMaxLevel) - prefixLength;
+    return (Key(1) << prefixLength) | (code >> nShifts);
+}
+
+inline unsigned decodePrefixLength(Key code) { return 63u - clz64(code); } // common.hpp:190-194
+
+inline Key decodePlaceholderBit(Key code) // common.hpp:205-213
+{
+    unsigned len = decodePrefixLength(code);
+    return (code ^ (Key(1) << len)) << (3 * kMaxLevel - len);
+}
+
+inline unsigned octalDigit(Key code, unsigned position) // common.hpp:298-302
+{
+    return unsigned(code >> (3u * (kMaxLevel - position))) & 7u;
+}
+
+inline int digitWeight(int digit) // common.hpp:318-322
+{
+    int fourGeqMask = -int(digit >= 4);
+    return ((7 - digit) & fourGeqMask) - (digit & ~fourGeqMask);
+}
+
+inline unsigned log8ceil(Key n) // common.hpp:118-124
+{
+    if (n == 0) return 0;
+    return kMaxLevel - (clz64(n - 1) - 1) / 3;
+}
+
+// ---------------------------------------------------------------------------
+// Hilbert curve (sfc/hilbert.hpp:43-94, 130-173, 259-275)
+Key iHilbert(unsigned px, unsigned py, unsigned pz)
+{
+    static const unsigned mortonToHilbert[8] = {0, 1, 3, 2, 7, 6, 4, 5};
+    Key key = 0;
+    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
+    {
+        unsigned xi = (px >> level) & 1u;
+        unsigned yi = (py >> level) & 1u;
+        unsigned zi = (pz >> level) & 1u;
+        unsigned octant = (xi << 2) | (yi << 1) | zi;
+        key = (key << 3) + mortonToHilbert[octant];
+
+        px ^= -(xi & ((!yi) | zi));
+        py ^= -((xi & (yi | zi)) | (yi & (!zi)));
+        pz ^= -((xi & (!yi) & (!zi)) | (yi & (!zi)));
+
+        if (zi)
+        {
+            unsigned t = px;
+            px = py, py = pz, pz = t;
+        }
+        else if (!yi)
+        {
+            unsigned t = px;
+            px = pz, pz = t;
+        }
+    }
+    return key;
+}
+
+void decodeHilbert(Key key, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (unsigned level = 0; level < kMaxLevel; ++level)
+    {
+        unsigned octant = unsigned(key >> (3 * level)) & 7u;
+        unsigned xi = octant >> 2u, yi = (octant >> 1u) & 1u, zi = octant & 1u;
+        if (yi ^ zi)
+        {
+            unsigned t = px;
+            px = pz, pz = py, py = t;
+        }
+        else if ((!xi & !yi & !zi) || (xi & yi & zi))
+        {
+            unsigned t = px;
+            px = pz, pz = t;
+        }
+        unsigned mask = (1u << level) - 1;
+        px ^= mask & (-(xi & (yi | zi)));
+        py ^= mask & (-((xi & ((!yi) | (!zi))) | ((!xi) & yi & zi)));
+        pz ^= mask & (-((xi & (!yi) & (!zi)) | (yi & zi)));
+        px |= (xi << level);
+        py |= ((xi ^ yi) << level);
+        pz |= ((yi ^ zi) << level);
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+// sfc3D, sfc/sfc.hpp:141-178: ix = int(floor(x*mx) - xmin*mx), clamped above
+Key sfc3D(double x, double y, double z, const Box& box)
+{
+    const int    mcoord     = (1 << kMaxLevel) - 1;
+    const double cubeLength = double(1u << kMaxLevel);
+    double mx = cubeLength * box.ilen[0], my = cubeLength * box.ilen[1], mz = cubeLength * box.ilen[2];
+    int ix = int(std::floor(x * mx) - box.lo[0] * mx);
+    int iy = int(std::floor(y * my) - box.lo[1] * my);
+    int iz = int(std::floor(z * mz) - box.lo[2] * mz);
+    ix = std::min(ix, mcoord);
+    iy = std::min(iy, mcoord);
+    iz = std::min(iz, mcoord);
+    return iHilbert(unsigned(ix), unsigned(iy), unsigned(iz));
+}
+
+// ---------------------------------------------------------------------------
+// cornerstone leaves (tree/csarray.hpp)
+unsigned calculateNodeCount(Key a, Key b, const Key* k0, const Key* k1, size_t maxCount) // :77-86
+{
+    size_t c = std::lower_bound(k0, k1, b) - std::lower_bound(k0, k1, a);
+    return unsigned(std::min(c, maxCount));
+}
+
+// computeNodeCounts without guesses (:182-235); the guessed variant returns the same counts
+void computeNodeCounts(const Key* tree, unsigned* counts, int nNodes, const Key* keys, size_t n, unsigned maxCount)
+{
+    int firstNode = nNodes, lastNode = nNodes;
+    if (n > 0)
+    {
+        firstNode = int(std::upper_bound(tree, tree + nNodes, keys[0]) - tree) - 1;
+        lastNode  = int(std::upper_bound(tree, tree + nNodes, keys[n - 1]) - tree);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nNodes; ++i)
+    {
+        if (i < firstNode || i >= lastNode) { counts[i] = 0; }
+        else { counts[i] = calculateNodeCount(tree[i], tree[i + 1], keys, keys + n, maxCount); }
+    }
+}
+
+// siblingAndLevel (:251-264)
+void siblingAndLevel(const Key* t, int i, int& sib, unsigned& level)
+{
+    Key thisNode = t[i];
+    level        = treeLevel(t[i + 1] - thisNode);
+    if (level == 0)
+    {
+        sib = -1;
+        return;
+    }
+    sib = int(octalDigit(thisNode, level));
+    bool siblings = (t[i - sib + 8] == t[i - sib] + nodeRange(level - 1));
+    if (!siblings) sib = -1;
+}
+
+// calculateNodeOp (:268-290)
+int calculateNodeOp(const Key* t, int i, const unsigned* counts, unsigned bucket)
+{
+    int sib;
+    unsigned level;
+    siblingAndLevel(t, i, sib, level);
+    if (sib > 0)
+    {
+        const unsigned* g = counts + i - sib;
+        size_t parentCount = 0;
+        for (int k = 0; k < 8; ++k)
+            parentCount += size_t(g[k]);
+        if (parentCount <= size_t(bucket)) return 0;
+    }
+    if (counts[i] > bucket * 512 && level + 3 < kMaxLevel) return 4096;
+    if (counts[i] > bucket * 64 && level + 2 < kMaxLevel) return 512;
+    if (counts[i] > bucket * 8 && level + 1 < kMaxLevel) return 64;
+    if (counts[i] > bucket && level < kMaxLevel) return 8;
+    return 1;
+}
+
+// updateOctree (:409-426) = rebalanceDecision (:310-329) + rebalanceTree (:376-389) + counts
+bool updateOctree(const Key* keys, size_t n, unsigned bucket, std::vector<Key>& tree, std::vector<unsigned>& counts,
+                  unsigned maxCount)
+{
+    int nNodes = int(tree.size()) - 1;
+    std::vector<int> ops(nNodes + 1);
+    bool converged = true;
+#pragma omp parallel for schedule(static) reduction(&& : converged)
+    for (int i = 0; i < nNodes; ++i)
+    {
+        ops[i] = calculateNodeOp(tree.data(), i, counts.data(), bucket);
+        converged = converged && (ops[i] == 1);
+    }
+    ops[nNodes] = 0;
+    // exclusive scan
+    int run = 0;
+    for (int i = 0; i <= nNodes; ++i)
+    {
+        int v = ops[i];
+        ops[i] = run;
+        run += v;
+    }
+    std::vector<Key> newTree(ops[nNodes] + 1);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nNodes; ++i) // processNode (:340-366)
+    {
+        Key thisNode = tree[i];
+        unsigned level = treeLevel(tree[i + 1] - thisNode);
+        int opCode = ops[i + 1] - ops[i];
+        int out = ops[i];
+        if (opCode == 1) { newTree[out] = thisNode; }
+        else if (opCode == 8)
+        {
+            for (int s = 0; s < 8; ++s)
+                newTree[out + s] = thisNode + Key(s) * nodeRange(level + 1);
+        }
+        else if (opCode > 8)
+        {
+            unsigned levelDiff = log8ceil(Key(opCode));
+            for (int s = 0; s < opCode; ++s)
+                newTree[out + s] = thisNode + Key(s) * nodeRange(level + levelDiff);
+        }
+    }
+    newTree.back() = tree.back();
+    tree.swap(newTree);
+    counts.resize(tree.size() - 1);
+    computeNodeCounts(tree.data(), counts.data(), int(tree.size()) - 1, keys, n, maxCount);
+    return converged;
+}
+
+// ---------------------------------------------------------------------------
+// internal octree (tree/octree.hpp:56-196)
+int binaryKeyWeight(Key key, unsigned level) // :56-65
+{
+    int ret = 0;
+    for (unsigned l = 1; l <= level + 1; ++l)
+        ret += digitWeight(int(octalDigit(key, l)));
+    return ret;
+}
+
+// ---------------------------------------------------------------------------
+// SPH helpers
+inline double lookup(const double* table, double v) // sph/table_lookup.hpp:13-26
+{
+    constexpr int    numIntervals = kTableSize - 1;
+    constexpr double support      = 2.0;
+    constexpr double dx           = support / numIntervals;
+    constexpr double invDx        = 1.0 / dx;
+    int    idx        = int(v * invDx);
+    double derivative = (idx >= numIntervals) ? 0.0 : (table[idx + 1] - table[idx]) * invDx;
+    return (idx >= numIntervals) ? 0.0 : table[idx] + derivative * (v - double(idx) * dx);
+}
+
+// applyPBC legacy threshold fold (sfc/box.hpp:217-238)
+inline void applyPBC(const Box& box, double r, double& xx, double& yy, double& zz)
+{
+    double* c[3] = {&xx, &yy, &zz};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (box.bc[d] != SPHX_BC_PERIODIC) continue;
+        if (*c[d] > r) *c[d] -= box.len[d];
+        else if (*c[d] < -r) *c[d] += box.len[d];
+    }
+}
+
+inline double wharmonic(double v) // sph/kernels.hpp:36-43
+{
+    if (v == 0.0) return 1.0;
+    const double Pv = M_PI_2 * v;
+    return std::sin(Pv) / Pv;
+}
+
+double simpson(double a, double b, uint64_t n, double sincIndex) // sph_kernel_tables.hpp:28-57
+{
+    auto f = [sincIndex](double x) { return 4.0 * M_PI * x * x * std::pow(wharmonic(x), sincIndex); };
+    uint64_t numOdd  = n / 2;
+    uint64_t numEven = (numOdd >= 1) ? numOdd - 1 : 0;
+    double   h       = (b - a) / double(n);
+    std::vector<double> odd(numOdd), even(numEven);
+    for (uint64_t i = 0; i < numOdd; ++i)
+        odd[i] = f(a + double(2 * (i + 1) - 1) * h);
+    for (uint64_t i = 0; i < numEven; ++i)
+        even[i] = f(a + double(2 * (i + 1)) * h);
+    std::sort(odd.begin(), odd.end());
+    std::sort(even.begin(), even.end());
+    double so = 0.0, se = 0.0;
+    for (double v : odd)
+        so += v;
+    for (double v : even)
+        se += v;
+    return h / 3.0 * (f(a) + f(b) + 4.0 * so + 2.0 * se);
+}
+
+} // namespace
+
+extern "C"
+{
+
+// --------------------------------------------------------------------------- tables
+// ParticlesData::createTables (particles_data.hpp:377-387) with HydroType = double and float
+void orc_kernel_tables(double sincIndex, double* wh64, float* wh32, double* K)
+{
+    if (K) *K = 1.0 / simpson(0.0, 2.0, 2000, sincIndex); // kernel_3D_k, sph_kernel_tables.hpp:78-85
+    const int numIntervals = kTableSize - 1;              // tabulateFunction, :88-102
+    if (wh64)
+    {
+        const double dx = (2.0 - 0.0) / numIntervals;
+        for (int i = 0; i < kTableSize; ++i)
+        {
+            double v = 0.0 + double(size_t(i)) * dx;
+            wh64[i]  = std::pow(wharmonic(v), sincIndex);
+        }
+    }
+    if (wh32)
+    {
+        const float dx = float((2.0 - 0.0) / numIntervals);
+        for (int i = 0; i < kTableSize; ++i)
+        {
+            float v = float(0.0 + double(float(size_t(i)) * dx));
+            wh32[i] = float(std::pow(wharmonic(double(v)), sincIndex));
+        }
+    }
+}
+
+// sphynx_3D_k (sph_kernel_tables.hpp:64-75)
+double orc_sphynx_k(double n)
+{
+    double b0 = 2.7012593e-2, b1 = 2.0410827e-2, b2 = 3.7451957e-3, b3 = 4.7013839e-2;
+    return b0 + b1 * std::sqrt(n) + b2 * n + b3 * std::sqrt(n * n * n);
+}
+
+double orc_simpson_sin(double a, double b, uint64_t n) // util::simpson on sin, for the Simpson known answer
+{
+    double h = (b - a) / double(n);
+    uint64_t numOdd = n / 2, numEven = numOdd >= 1 ? numOdd - 1 : 0;
+    std::vector<double> odd(numOdd), even(numEven);
+    for (uint64_t i = 0; i < numOdd; ++i)
+        odd[i] = std::sin(a + double(2 * (i + 1) - 1) * h);
+    for (uint64_t i = 0; i < numEven; ++i)
+        even[i] = std::sin(a + double(2 * (i + 1)) * h);
+    std::sort(odd.begin(), odd.end());
+    std::sort(even.begin(), even.end());
+    double so = 0, se = 0;
+    for (double v : odd)
+        so += v;
+    for (double v : even)
+        se += v;
+    return h / 3.0 * (std::sin(a) + std::sin(b) + 4.0 * so + 2.0 * se);
+}
+
+double orc_lookup(const double* table, double v) { return lookup(table, v); }
+
+// --------------------------------------------------------------------------- keys
+uint64_t orc_ihilbert(unsigned x, unsigned y, unsigned z) { return iHilbert(x, y, z); }
+
+void orc_decode_hilbert(uint64_t key, unsigned* xyz) { decodeHilbert(key, xyz[0], xyz[1], xyz[2]); }
+
+// computeSfcKeys (sfc/sfc.hpp:267-275)
+void orc_compute_sfc_keys(const double* x, const double* y, const double* z, uint64_t* keys, uint64_t n,
+                          const sphx_box* b)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; ++i)
+        if (keys[i] != kRemoveKey) keys[i] = sfc3D(x[i], y[i], z[i], box);
+}
+
+// sort_by_key (primitives/gather.hpp:42-67): std::stable_sort of (key, value) pairs
+void orc_sort_by_key(uint64_t* keys, uint32_t* vals, uint64_t n)
+{
+    std::vector<std::pair<uint64_t, uint32_t>> kv(n);
+    for (uint64_t i = 0; i < n; ++i)
+        kv[i] = {keys[i], vals[i]};
+    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (uint64_t i = 0; i < n; ++i)
+        keys[i] = kv[i].first, vals[i] = kv[i].second;
+}
+
+// --------------------------------------------------------------------------- octree
+// computeOctree (tree/csarray.hpp:430-440).  Returns the number of leaves L, or -(L)
+// if cap < L + 1 (nothing written then).
+int orc_compute_octree(const uint64_t* keys, uint64_t n, unsigned bucket, unsigned maxCount, uint64_t* leaves,
+                       unsigned* counts, int cap)
+{
+    std::vector<Key> tree{0, kRootRange};
+    std::vector<unsigned> cnt{unsigned(n)};
+    while (!updateOctree(keys, n, bucket, tree, cnt, maxCount))
+        ;
+    int L = int(tree.size()) - 1;
+    if (cap < L + 1) return -L;
+    std::copy(tree.begin(), tree.end(), leaves);
+    std::copy(cnt.begin(), cnt.end(), counts);
+    return L;
+}
+
+// single rebalance step on a given tree (updateOctree, csarray.hpp:409-426); returns
+// the converged flag; tree/counts capacities must be >= the new sizes (cap).
+int orc_update_octree(const uint64_t* keys, uint64_t n, unsigned bucket, uint64_t* tree, unsigned* counts,
+                      int* numNodes, int cap)
+{
+    std::vector<Key> t(tree, tree + *numNodes + 1);
+    std::vector<unsigned> c(counts, counts + *numNodes);
+    bool conv = updateOctree(keys, n, bucket, t, c, ~0u);
+    if (int(t.size()) > cap) return -1;
+    std::copy(t.begin(), t.end(), tree);
+    std::copy(c.begin(), c.end(), counts);
+    *numNodes = int(t.size()) - 1;
+    return conv ? 1 : 0;
+}
+
+// rebalanceDecision (csarray.hpp:310-329) on its own, for the known-answer tests
+int orc_rebalance_decision(const uint64_t* tree, const unsigned* counts, int nNodes, unsigned bucket, int* ops)
+{
+    int conv = 1;
+    for (int i = 0; i < nNodes; ++i)
+    {
+        ops[i] = calculateNodeOp(tree, i, counts, bucket);
+        if (ops[i] != 1) conv = 0;
+    }
+    return conv;
+}
+
+// buildOctreeCpu (tree/octree.hpp:168-196)
+void orc_build_octree(const uint64_t* leaves, int numLeaves, uint64_t* prefixes, int* childOffsets, int* parents,
+                      int* levelRange, int* internalToLeaf, int* leafToInternal)
+{
+    int numInternal = (numLeaves - 1) / 7;
+    int numNodes    = numLeaves + numInternal;
+    // createUnsortedLayoutCpu (:78-100)
+#pragma omp parallel for schedule(static)
+    for (int tid = 0; tid < numLeaves; ++tid)
+    {
+        Key key = leaves[tid];
+        unsigned level = treeLevel(leaves[tid + 1] - key);
+        prefixes[tid + numInternal]       = encodePlaceholderBit(key, 3 * int(level));
+        internalToLeaf[tid + numInternal] = tid + numInternal;
+        unsigned prefixLength = unsigned(commonPrefix(key, leaves[tid + 1]));
+        if (prefixLength % 3 == 0 && tid < numLeaves - 1)
+        {
+            int octIndex = (tid + binaryKeyWeight(key, prefixLength / 3)) / 7;
+            prefixes[octIndex]       = encodePlaceholderBit(key, int(prefixLength));
+            internalToLeaf[octIndex] = octIndex;
+        }
+    }
+    // sort_by_key(prefixes, internalToLeaf)
+    orc_sort_by_key(prefixes, reinterpret_cast<uint32_t*>(internalToLeaf), uint64_t(numNodes));
+    for (int i = 0; i < numNodes; ++i)
+        leafToInternal[internalToLeaf[i]] = i;
+    for (int i = 0; i < numNodes; ++i)
+        internalToLeaf[i] -= numInternal;
+    // getLevelRangeCpu (:152-160)
+    for (unsigned level = 0; level <= kMaxLevel; ++level)
+        levelRange[level] =
+            int(std::lower_bound(prefixes, prefixes + numNodes, encodePlaceholderBit(0, 3 * int(level))) - prefixes);
+    levelRange[kMaxLevel + 1] = numNodes;
+    std::fill(childOffsets, childOffsets + numNodes, 0);
+    // linkTreeCpu (:115-148)
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < numInternal; ++i)
+    {
+        int idxA = leafToInternal[i];
+        Key prefix = prefixes[idxA];
+        Key nodeKey = decodePlaceholderBit(prefix);
+        unsigned prefixLength = decodePrefixLength(prefix);
+        unsigned level = prefixLength / 3;
+        Key childPrefix = encodePlaceholderBit(nodeKey, int(prefixLength) + 3);
+        int s = levelRange[level + 1], e = levelRange[level + 2];
+        int childIdx = int(std::lower_bound(prefixes + s, prefixes + e, childPrefix) - prefixes);
+        if (childIdx != e && childPrefix == prefixes[childIdx])
+        {
+            childOffsets[idxA] = childIdx;
+            parents[(childIdx - 1) / 8] = idxA;
+        }
+    }
+}
+
+// nodeFpCenters (focus/source_center.hpp:146-158) with hilbertIBox (hilbert.hpp:259-275)
+// and centerAndSize (box.hpp:317-334); centers/sizes are xyz-interleaved
+void orc_node_centers(const uint64_t* prefixes, int numNodes, const sphx_box* b, double* centers, double* sizes)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < numNodes; ++i)
+    {
+        Key prefix = prefixes[i];
+        Key startKey = decodePlaceholderBit(prefix);
+        unsigned level = decodePrefixLength(prefix) / 3;
+        unsigned cubeLength = (1u << kMaxLevel) >> level;
+        unsigned mask = ~(cubeLength - 1);
+        unsigned ix, iy, iz;
+        decodeHilbert(startKey, ix, iy, iz);
+        int imin[3] = {int(ix & mask), int(iy & mask), int(iz & mask)};
+        for (int d = 0; d < 3; ++d)
+        {
+            int imax = imin[d] + int(cubeLength);
+            const double uL = 1.0 / double(1u << kMaxLevel);
+            double halfUnit = 0.5 * uL * box.len[d];
+            centers[3 * i + d] = box.lo[d] + double(imax + imin[d]) * halfUnit;
+            sizes[3 * i + d]   = double(imax - imin[d]) * halfUnit;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- neighbors
+// findNeighbors of one particle (findneighbors.hpp:77-147) with singleTraversal
+// (traversal/traversal.hpp:26-69) and minDistance (traversal/boxoverlap.hpp:230-250)
+static unsigned findNeighborsOne(uint32_t i, const double* x, const double* y, const double* z, const double* h,
+                                 const int* childOffsets, const int* parents, const int* internalToLeaf,
+                                 const uint32_t* layout, const double* centers, const double* sizes,
+                                 const Box& box, unsigned ngmax, uint32_t* nbrs)
+{
+    double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+    double radiusSq = 4.0 * hi * hi;
+    double cellRadiusSq = radiusSq * 1.0f * 1.0f; // searchExtFactor = 1.0
+    unsigned nn = 0;
+    bool anyPbc = box.bc[0] == SPHX_BC_PERIODIC || box.bc[1] == SPHX_BC_PERIODIC || box.bc[2] == SPHX_BC_PERIODIC;
+    double p[3] = {xi, yi, zi};
+    bool inside = true; // insideBox (boxoverlap.hpp:218-227)
+    for (int d = 0; d < 3; ++d)
+    {
+        double lo = p[d] - 2.0 * hi, hi2 = p[d] + 2.0 * hi;
+        inside = inside && lo >= box.lo[d] && hi2 <= box.hi[d];
+    }
+    bool usePbc = anyPbc && !inside;
+
+    auto overlaps = [&](int idx)
+    {
+        double dist[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            double dX = centers[3 * idx + d] - p[d];
+            if (usePbc) // applyPbc (box.hpp:177-189)
+                dX -= double(box.bc[d] == SPHX_BC_PERIODIC) * box.len[d] * std::rint(dX * box.ilen[d]);
+            dX = std::fabs(dX);
+            dX -= sizes[3 * idx + d];
+            dX += std::fabs(dX);
+            dX *= 0.5;
+            dist[d] = dX;
+        }
+        return dist[0] * dist[0] + dist[1] * dist[1] + dist[2] * dist[2] < cellRadiusSq;
+    };
+    auto search = [&](int idx)
+    {
+        int leafIdx = internalToLeaf[idx];
+        for (uint32_t j = layout[leafIdx]; j < layout[leafIdx + 1]; ++j)
+        {
+            if (j == i) continue;
+            double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+            if (usePbc) // distanceSq<true> (findneighbors.hpp:33-48)
+            {
+                dx -= double(box.bc[0] == SPHX_BC_PERIODIC) * box.len[0] * std::rint(dx * box.ilen[0]);
+                dy -= double(box.bc[1] == SPHX_BC_PERIODIC) * box.len[1] * std::rint(dy * box.ilen[1]);
+                dz -= double(box.bc[2] == SPHX_BC_PERIODIC) * box.len[2] * std::rint(dz * box.ilen[2]);
+            }
+            if (dx * dx + dy * dy + dz * dz < radiusSq)
+            {
+                if (nn < ngmax && nbrs) nbrs[nn] = j;
+                nn++;
+            }
+        }
+    };
+    // singleTraversal
+    if (!overlaps(0)) return 0;
+    if (childOffsets[0] == 0)
+    {
+        search(0);
+        return nn;
+    }
+    int node = childOffsets[0];
+    bool backtrack = false;
+    while (node != 0)
+    {
+        bool isLeaf = childOffsets[node] == 0;
+        bool descend = !backtrack && overlaps(node);
+        if (isLeaf && descend) search(node);
+        int siblingIdx = (node - 1) % 8;
+        if (!isLeaf && descend)
+        {
+            node = childOffsets[node];
+            backtrack = false;
+        }
+        else if (siblingIdx < 7)
+        {
+            node++;
+            backtrack = false;
+        }
+        else
+        {
+            node = parents[(node - 1) / 8];
+            backtrack = true;
+        }
+    }
+    return nn;
+}
+
+// findNeighbors batch (findneighbors.hpp:149-170); nbrs may be NULL (counts only)
+void orc_find_neighbors(const double* x, const double* y, const double* z, const double* h, uint32_t first,
+                        uint32_t last, const sphx_box* b, const int* childOffsets, const int* parents,
+                        const int* internalToLeaf, const uint32_t* layout, const double* centers,
+                        const double* sizes, unsigned ngmax, uint32_t* nbrs, unsigned* nc)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(last - first); ++i)
+    {
+        uint32_t id = uint32_t(i) + first;
+        nc[i] = findNeighborsOne(id, x, y, z, h, childOffsets, parents, internalToLeaf, layout, centers, sizes, box,
+                                 ngmax, nbrs ? nbrs + size_t(i) * ngmax : nullptr);
+    }
+}
+
+// updateH (sph/kernels.hpp:27-33)
+double orc_update_h(unsigned ng0, unsigned nc, double h)
+{
+    const double c0 = 1023.0, ex = 1.0 / 10.0;
+    return h * 0.5 * std::pow(1.0 + c0 * ng0 / double(nc), ex);
+}
+
+// findNeighborsSph (sph/include/sph/find_neighbors.hpp:10-41); h is updated in place
+void orc_find_neighbors_sph(const double* x, const double* y, const double* z, double* h, uint32_t first,
+                            uint32_t last, const sphx_box* b, const int* childOffsets, const int* parents,
+                            const int* internalToLeaf, const uint32_t* layout, const double* centers,
+                            const double* sizes, unsigned ng0, unsigned ngmax, uint32_t* nbrs, unsigned* nc)
+{
+    Box box = makeBox(b);
+    unsigned ngmin = ng0 / 4;
+    const int maxIteration = 10;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(last - first); ++i)
+    {
+        uint32_t id = uint32_t(i) + first;
+        uint32_t* nb = nbrs + size_t(i) * ngmax;
+        unsigned ncSph = 1 + findNeighborsOne(id, x, y, z, h, childOffsets, parents, internalToLeaf, layout, centers,
+                                              sizes, box, ngmax, nb);
+        int iteration = 0;
+        while ((ngmin > ncSph || (ncSph - 1) > ngmax) && iteration++ < maxIteration)
+        {
+            h[id] = orc_update_h(ng0, ncSph, h[id]);
+            ncSph = 1 + findNeighborsOne(id, x, y, z, h, childOffsets, parents, internalToLeaf, layout, centers,
+                                         sizes, box, ngmax, nb);
+        }
+        nc[i] = ncSph;
+        if (iteration == maxIteration && (ngmin > ncSph || (ncSph - 1) > ngmax))
+        {
+            h[i]  = 0.0; // the reference indexes with the local i here (find_neighbors.hpp:37)
+            nc[i] = 1;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- SPH loops (T = double)
+// computeDensityImpl -> computeXMassImpl -> xmassJLoop (hydro_std/density.hpp:41-53,
+// hydro_ve/xmass.hpp:39-67, hydro_ve/xmass_kern.hpp:50-79); rho = m / xm
+void orc_density(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                 const double* x, const double* y, const double* z, const double* h, const double* m,
+                 const double* wh, double K, const sphx_box* b, double* rho)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + ngmax * ni;
+        double xi = x[i], yi = y[i], zi = z[i], hi = h[i], mi = m[i];
+        double hInv = 1.0 / hi;
+        double h3Inv = hInv * hInv * hInv;
+        double rho0i = mi;
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double xx = xi - x[j], yy = yi - y[j], zz = zi - z[j]; // distancePBC (box.hpp:240-250)
+            applyPBC(box, 2.0 * hi, xx, yy, zz);
+            double dist = std::sqrt(xx * xx + yy * yy + zz * zz);
+            double vloc = dist * hInv;
+            double w = lookup(wh, vloc);
+            rho0i += w * m[j];
+        }
+        double xm = mi / (rho0i * K * h3Inv);
+        rho[i] = m[i] / xm;
+    }
+}
+
+// idealGasEOS_u over a range (hydro_std/eos.hpp:52-78, sph/eos.hpp:34-44)
+void orc_eos(uint32_t first, uint32_t last, const double* u, const double* rho, double gamma, double* p, double* c)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = first; i < int64_t(last); ++i)
+    {
+        double tmp = u[i] * (gamma - 1.0);
+        p[i] = rho[i] * tmp;
+        c[i] = std::sqrt(gamma * tmp);
+    }
+}
+
+// IADJLoopSTD (hydro_std/iad_kern.hpp:12-78) over [first,last) with capped counts (iad.hpp:64-70)
+static void iadOne(size_t i, unsigned cnt, const uint32_t* nb, const double* x, const double* y, const double* z,
+                   const double* h, const double* m, const double* rho, const double* wh, double K, const Box& box,
+                   double* c11, double* c12, double* c13, double* c22, double* c23, double* c33)
+{
+    double t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
+    double xi = x[i], yi = y[i], zi = z[i], hi = h[i], hiInv = 1.0 / hi;
+    for (unsigned pj = 0; pj < cnt; ++pj)
+    {
+        uint32_t j = nb[pj];
+        double rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        applyPBC(box, 2.0 * hi, rx, ry, rz);
+        double dist = std::sqrt(rx * rx + ry * ry + rz * rz);
+        double vloc = dist * hiInv;
+        double w = lookup(wh, vloc);
+        double mj_roj_w = m[j] / rho[j] * w;
+        t11 += rx * rx * mj_roj_w;
+        t12 += rx * ry * mj_roj_w;
+        t13 += rx * rz * mj_roj_w;
+        t22 += ry * ry * mj_roj_w;
+        t23 += ry * rz * mj_roj_w;
+        t33 += rz * rz * mj_roj_w;
+    }
+    auto getExp = [](double v) { return v == 0.0 ? 0 : std::ilogb(v); };
+    int expSum = getExp(t11) + getExp(t12) + getExp(t13) + getExp(t22) + getExp(t23) + getExp(t33);
+    double norm = std::ldexp(1.0, -expSum / 6);
+    t11 *= norm, t12 *= norm, t13 *= norm, t22 *= norm, t23 *= norm, t33 *= norm;
+    double det = t11 * t22 * t33 + 2.0 * t12 * t23 * t13 - t11 * t23 * t23 - t22 * t13 * t13 - t33 * t12 * t12;
+    double factor = norm * (hi * hi * hi) / (det * K);
+    if (std::isnan(factor) && cnt == 0) factor = 0.0;
+    c11[i] = (t22 * t33 - t23 * t23) * factor;
+    c12[i] = (t13 * t23 - t33 * t12) * factor;
+    c13[i] = (t12 * t23 - t22 * t13) * factor;
+    c22[i] = (t11 * t33 - t13 * t13) * factor;
+    c23[i] = (t13 * t12 - t11 * t23) * factor;
+    c33[i] = (t11 * t22 - t12 * t12) * factor;
+}
+
+void orc_iad(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax, const double* x,
+             const double* y, const double* z, const double* h, const double* m, const double* rho, const double* wh,
+             double K, const sphx_box* b, double* c11, double* c12, double* c13, double* c22, double* c23,
+             double* c33)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        iadOne(i, cnt, nbrs + ngmax * ni, x, y, z, h, m, rho, wh, K, box, c11, c12, c13, c22, c23, c33);
+    }
+}
+
+// single-particle IAD with an explicit neighbor list (the std.cpp known-answer form)
+void orc_iad_one(uint32_t i, const uint32_t* nb, unsigned cnt, const double* x, const double* y, const double* z,
+                 const double* h, const double* m, const double* rho, const double* wh, double K, const sphx_box* b,
+                 double* out6)
+{
+    Box box = makeBox(b);
+    std::vector<double> t(6 * (size_t(i) + 1));
+    size_t s = size_t(i) + 1;
+    iadOne(i, cnt, nb, x, y, z, h, m, rho, wh, K, box, &t[0], &t[s], &t[2 * s], &t[3 * s], &t[4 * s], &t[5 * s]);
+    for (int k = 0; k < 6; ++k)
+        out6[k] = t[k * s + i];
+}
+
+// momentumAndEnergyJLoop (hydro_std/momentum_energy_kern.hpp:12-134)
+static void meOne(size_t i, unsigned cnt, const uint32_t* nb, const double* x, const double* y, const double* z,
+                  const double* vx, const double* vy, const double* vz, const double* h, const double* m,
+                  const double* rho, const double* p, const double* c, const double* c11, const double* c12,
+                  const double* c13, const double* c22, const double* c23, const double* c33, const double* wh,
+                  double K, const Box& box, double* ax, double* ay, double* az, double* du, double* maxvsignal)
+{
+    const double gradh_i = 1.0, gradh_j = 1.0;
+    double xi = x[i], yi = y[i], zi = z[i], vxi = vx[i], vyi = vy[i], vzi = vz[i];
+    double hi = h[i], roi = rho[i], pri = p[i], ci = c[i];
+    double mi_roi = m[i] / rho[i];
+    double hiInv = 1.0 / hi, hiInv3 = hiInv * hiInv * hiInv;
+    double maxv = 0.0, mx = 0.0, my = 0.0, mz = 0.0, energy = 0.0;
+    double c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
+    for (unsigned pj = 0; pj < cnt; ++pj)
+    {
+        uint32_t j = nb[pj];
+        double rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        applyPBC(box, 2.0 * hi, rx, ry, rz);
+        double r2 = rx * rx + ry * ry + rz * rz;
+        double dist = std::sqrt(r2);
+        double vx_ij = vxi - vx[j], vy_ij = vyi - vy[j], vz_ij = vzi - vz[j];
+        double hj = h[j], hjInv = 1.0 / hj;
+        double v1 = dist * hiInv, v2 = dist * hjInv;
+        double rv = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+        double hjInv3 = hjInv * hjInv * hjInv;
+        double Wi = hiInv3 * lookup(wh, v1);
+        double Wj = hjInv3 * lookup(wh, v2);
+        double tA1i = c11i * rx + c12i * ry + c13i * rz;
+        double tA2i = c12i * rx + c22i * ry + c23i * rz;
+        double tA3i = c13i * rx + c23i * ry + c33i * rz;
+        double tA1j = c11[j] * rx + c12[j] * ry + c13[j] * rz;
+        double tA2j = c12[j] * rx + c22[j] * ry + c23[j] * rz;
+        double tA3j = c13[j] * rx + c23[j] * ry + c33[j] * rz;
+        double roj = rho[j], cj = c[j];
+        double wij = rv / dist;
+        // artificial_viscosity (kernels.hpp:71-85) with alpha_i = alpha_j = 1, beta = 2
+        double visc = 0.0;
+        if (wij < 0.0)
+        {
+            double vij_signal = (1.0 + 1.0) * 0.25 * (ci + cj) - 2.0 * wij;
+            visc = -vij_signal * wij;
+        }
+        double viscosity_ij = 0.5 * visc;
+        double vijsignal = ci + cj - 3.0 * wij;
+        maxv = (vijsignal > maxv) ? vijsignal : maxv;
+        double mj = m[j];
+        double mj_roj_Wj = mj / roj * Wj;
+        double mj_pro_i = mj * pri / (gradh_i * roi * roi);
+        {
+            double a = Wi * (mj_pro_i + viscosity_ij * mi_roi);
+            double bb = mj_roj_Wj * (p[j] / (roj * gradh_j) + viscosity_ij);
+            mx += a * tA1i + bb * tA1j;
+            my += a * tA2i + bb * tA2j;
+            mz += a * tA3i + bb * tA3j;
+        }
+        {
+            double a = Wi * (2.0 * mj_pro_i + viscosity_ij * mi_roi);
+            double bb = viscosity_ij * mj_roj_Wj;
+            energy += vx_ij * (a * tA1i + bb * tA1j) + vy_ij * (a * tA2i + bb * tA2j) + vz_ij * (a * tA3i + bb * tA3j);
+        }
+    }
+    du[i] = -K * 0.5 * energy;
+    ax[i] = K * mx;
+    ay[i] = K * my;
+    az[i] = K * mz;
+    *maxvsignal = maxv;
+}
+
+// computeMomentumEnergyStdImpl (hydro_std/momentum_energy.hpp:41-94); returns minDtCourant
+double orc_momentum_energy(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                           const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                           const double* vz, const double* h, const double* m, const double* rho, const double* p,
+                           const double* c, const double* c11, const double* c12, const double* c13,
+                           const double* c22, const double* c23, const double* c33, const double* wh, double K,
+                           float Kcour, const sphx_box* b, double* ax, double* ay, double* az, double* du)
+{
+    Box box = makeBox(b);
+    double minDt = INFINITY;
+#pragma omp parallel for schedule(static) reduction(min : minDt)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        double maxv = 0;
+        meOne(i, cnt, nbrs + ngmax * ni, x, y, z, vx, vy, vz, h, m, rho, p, c, c11, c12, c13, c22, c23, c33, wh, K,
+              box, ax, ay, az, du, &maxv);
+        // tsKCourant (kernels.hpp:10-17)
+        double v = maxv > 0.0 ? maxv : c[i];
+        double dt = h[i] > 0.0 ? double(Kcour * h[i] / v) : INFINITY;
+        minDt = std::min(minDt, dt);
+    }
+    return minDt;
+}
+
+// single-particle momentum-energy with an explicit list (std.cpp known-answer form);
+// out5 = {grad_Px, grad_Py, grad_Pz, du, maxvsignal}
+void orc_momentum_energy_one(uint32_t i, const uint32_t* nb, unsigned cnt, const double* x, const double* y,
+                             const double* z, const double* vx, const double* vy, const double* vz, const double* h,
+                             const double* m, const double* rho, const double* p, const double* c,
+                             const double* c11, const double* c12, const double* c13, const double* c22,
+                             const double* c23, const double* c33, const double* wh, double K, const sphx_box* b,
+                             double* out5)
+{
+    Box box = makeBox(b);
+    size_t s = size_t(i) + 1;
+    std::vector<double> t(4 * s);
+    meOne(i, cnt, nb, x, y, z, vx, vy, vz, h, m, rho, p, c, c11, c12, c13, c22, c23, c33, wh, K, box, &t[0], &t[s],
+          &t[2 * s], &t[3 * s], &out5[4]);
+    for (int k = 0; k < 4; ++k)
+        out5[k] = t[k * s + i];
+}
+
+// single-particle xmass with an explicit list (ve.cpp XMass known-answer form)
+double orc_xmass_one(uint32_t i, const uint32_t* nb, unsigned cnt, const double* x, const double* y, const double* z,
+                     const double* h, const double* m, const double* wh, double K, const sphx_box* b)
+{
+    Box box = makeBox(b);
+    double xi = x[i], yi = y[i], zi = z[i], hi = h[i], mi = m[i];
+    double hInv = 1.0 / hi, h3Inv = hInv * hInv * hInv;
+    double rho0i = mi;
+    for (unsigned pj = 0; pj < cnt; ++pj)
+    {
+        uint32_t j = nb[pj];
+        double xx = xi - x[j], yy = yi - y[j], zz = zi - z[j];
+        applyPBC(box, 2.0 * hi, xx, yy, zz);
+        double dist = std::sqrt(xx * xx + yy * yy + zz * zz);
+        rho0i += lookup(wh, dist * hInv) * m[j];
+    }
+    return mi / (rho0i * K * h3Inv);
+}
+
+} // extern "C"

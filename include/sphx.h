@@ -67,7 +67,7 @@ typedef struct sphx_octree
     int32_t   numLeafNodes;
     int32_t   numInternalNodes; /* (numLeafNodes - 1) / 7                                */
     int32_t   numNodes;         /* numLeafNodes + numInternalNodes                        */
-    int32_t   maxDepth;         /* deepest level with nodes (filled by sphx_build_octree) */
+    int32_t   reserved;
     uint64_t* prefixes;         /* [numNodes] Warren-Salmon placeholder-bit keys          */
     int32_t*  childOffsets;     /* [numNodes + 1], 0 = leaf                               */
     int32_t*  parents;          /* [max(1, (numNodes - 1) / 8)]                           */
@@ -126,6 +126,14 @@ int         sphx_ctx_set_stream(sphx_ctx* ctx, void* stream);
 int         sphx_ctx_synchronize(sphx_ctx* ctx);
 const char* sphx_last_error(const sphx_ctx* ctx);
 const char* sphx_version(void);
+
+/* context options */
+enum
+{
+    SPHX_OPT_FP64_EXACT = 1 /* 1: FP64 particle loops compiled without FMA contraction, so
+                               every operation rounds like the reference CPU loops (default 0) */
+};
+int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value);
 
 /* ---- kernel tables ------------------------------------------------------- */
 /* Host-side table and normalisation: replaces ParticlesData::createTables

@@ -1,0 +1,506 @@
+// sphx_core.hip - context, scratch arena, device-wide scan, LSD radix sort,
+// gather and Hilbert keys.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sphx_internal.hpp"
+
+namespace sphx
+{
+
+int fail(sphx_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int failHip(sphx_ctx* ctx, hipError_t e, const char* what)
+{
+    return fail(ctx, SPHX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int argError(sphx_ctx* ctx, const char* what) { return fail(ctx, SPHX_ERR_ARG, what); }
+
+void* scratch(sphx_ctx* ctx, Slot s, size_t bytes)
+{
+    DevBuf& b = ctx->buf[s];
+    if (b.bytes >= bytes && b.p) return b.p;
+    if (b.p)
+    {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.p);
+        b.p     = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
+    if (hipMalloc(&b.p, want) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        fail(ctx, SPHX_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.p;
+}
+
+bool validBox(const sphx_box* b)
+{
+    if (!b) return false;
+    if (!(b->xmax > b->xmin) || !(b->ymax > b->ymin) || !(b->zmax > b->zmin)) return false;
+    int bc[3] = {b->bx, b->by, b->bz};
+    for (int v : bc)
+        if (v < 0 || v > 2) return false;
+    return true;
+}
+
+// ===========================================================================
+// exclusive scan (uint32), three-phase reduce-then-scan with recursion
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS   = 8;
+constexpr int SCAN_TILE    = SCAN_THREADS * SCAN_ITEMS; // 2048
+
+// block-wide exclusive scan of one value per thread (256 threads = 4 waves)
+__device__ inline uint32_t blockExclusiveScan256(uint32_t v, uint32_t* sWave, uint32_t& total)
+{
+    unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= unsigned(o)) incl += t;
+    }
+    if (lane == 63) sWave[wave] = incl;
+    __syncthreads();
+    uint32_t waveOfs = 0;
+    for (unsigned w = 0; w < wave; ++w)
+        waveOfs += sWave[w];
+    total = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+    __syncthreads();
+    return waveOfs + incl - v;
+}
+
+// logical input has n + 1 entries, in[n] := 0, so out[n] receives the total
+__global__ __launch_bounds__(SCAN_THREADS) void scanTiles(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                          uint32_t* tileSums)
+{
+    __shared__ uint32_t sWave[4];
+    uint64_t base = uint64_t(blockIdx.x) * SCAN_TILE + uint64_t(threadIdx.x) * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+    {
+        uint64_t i = base + k;
+        v[k]       = (i < n) ? in[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = blockExclusiveScan256(sum, sWave, total);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+    {
+        uint64_t i = base + k;
+        if (i <= n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0 && tileSums) tileSums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scanAddTileOffsets(uint32_t* out, uint64_t n,
+                                                                   const uint32_t* tileOfs)
+{
+    uint32_t add = tileOfs[blockIdx.x];
+    uint64_t base = uint64_t(blockIdx.x) * SCAN_TILE;
+    for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_THREADS)
+    {
+        uint64_t i = base + k;
+        if (i <= n) out[i] += add;
+    }
+}
+
+static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, int level)
+{
+    uint64_t numTiles = (n + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    if (numTiles == 1)
+    {
+        scanTiles<<<1, SCAN_THREADS, 0, ctx->stream>>>(in, out, n, nullptr);
+        SPHX_LAUNCH_CHECK(ctx, "scanTiles");
+        return SPHX_OK;
+    }
+    if (level > 2) return fail(ctx, SPHX_ERR_ARG, "scan too large");
+    Slot s = Slot(S_SCAN0 + level);
+    uint32_t* sums = scratchT<uint32_t>(ctx, s, numTiles + 1);
+    if (!sums) return SPHX_ERR_NOMEM;
+    scanTiles<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(in, out, n, sums);
+    SPHX_LAUNCH_CHECK(ctx, "scanTiles");
+    // exclusive scan of the tile sums in place (numTiles entries, total discarded into sums[numTiles])
+    SPHX_TRY(scanLevel(ctx, sums, sums, numTiles, level + 1));
+    scanAddTileOffsets<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(out, n, sums);
+    SPHX_LAUNCH_CHECK(ctx, "scanAddTileOffsets");
+    return SPHX_OK;
+}
+
+int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n)
+{
+    return scanLevel(ctx, in, out, n, 0);
+}
+
+// ===========================================================================
+// LSD radix sort, 8-bit digits.  Per pass: tile histograms -> scan -> stable
+// scatter through an LDS-staged local sort so that global writes are
+// contiguous runs per digit.
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS   = 16;
+constexpr int SORT_TILE    = SORT_THREADS * SORT_ITEMS; // 4096
+constexpr int RADIX        = 256;
+
+__global__ __launch_bounds__(SORT_THREADS) void radixUpsweep(const uint64_t* __restrict__ keys, uint64_t n,
+                                                             int shift, uint32_t* __restrict__ tileHist,
+                                                             uint32_t numTiles)
+{
+    __shared__ uint32_t h[4][RADIX];
+    unsigned wave = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < 4 * RADIX; d += SORT_THREADS)
+        (&h[0][0])[d] = 0;
+    __syncthreads();
+    uint64_t base = uint64_t(blockIdx.x) * SORT_TILE;
+#pragma unroll 4
+    for (int k = 0; k < SORT_ITEMS; ++k)
+    {
+        uint64_t i = base + uint64_t(k) * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[wave][unsigned(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    unsigned d = threadIdx.x;
+    tileHist[uint64_t(d) * numTiles + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+}
+
+template<bool HasVals>
+__global__ __launch_bounds__(SORT_THREADS) void radixScatter(const uint64_t* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin,
+                                                             uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                             uint64_t n, int shift,
+                                                             const uint32_t* __restrict__ tileOfs, uint32_t numTiles)
+{
+    __shared__ uint64_t sk[SORT_TILE];
+    __shared__ uint32_t sv[HasVals ? SORT_TILE : 1];
+    __shared__ uint32_t wcount[4][RADIX];
+    __shared__ uint32_t dStart[RADIX];
+    __shared__ uint32_t gOfs[RADIX];
+    __shared__ uint32_t sWave[4];
+
+    const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < 4 * RADIX; d += SORT_THREADS)
+        (&wcount[0][0])[d] = 0;
+    __syncthreads();
+
+    const uint64_t tileBase = uint64_t(blockIdx.x) * SORT_TILE;
+    const uint64_t waveBase = tileBase + uint64_t(wave) * 64 * SORT_ITEMS;
+    const uint64_t ltMask   = (uint64_t(1) << lane) - 1;
+
+    uint64_t key[SORT_ITEMS];
+    uint32_t val[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+
+    // ranks within the wave, in original order (item k, lane l) <-> element waveBase + 64 k + l
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k)
+    {
+        uint64_t i     = waveBase + uint64_t(k) * 64 + lane;
+        bool     valid = i < n;
+        key[k]         = valid ? kin[i] : 0;
+        if constexpr (HasVals) val[k] = valid ? vin[i] : 0;
+        unsigned d     = unsigned(key[k] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+        {
+            bool     bit = (d >> b) & 1u;
+            uint64_t bb  = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        unsigned below = unsigned(__popcll(peers & ltMask));
+        unsigned cnt   = unsigned(__popcll(peers));
+        uint32_t old   = valid ? wcount[wave][d] : 0u;
+        rank[k]        = old + below;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wcount[wave][d] = old + cnt;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // per digit: exclusive offsets across waves, tile totals, digit starts within the tile
+    {
+        unsigned d   = threadIdx.x;
+        uint32_t c0  = wcount[0][d], c1 = wcount[1][d], c2 = wcount[2][d], c3 = wcount[3][d];
+        uint32_t tot = c0 + c1 + c2 + c3;
+        __syncthreads();
+        wcount[0][d] = 0;
+        wcount[1][d] = c0;
+        wcount[2][d] = c0 + c1;
+        wcount[3][d] = c0 + c1 + c2;
+        uint32_t total;
+        uint32_t start = blockExclusiveScan256(tot, sWave, total);
+        dStart[d]      = start;
+        gOfs[d]        = tileOfs[uint64_t(d) * numTiles + blockIdx.x];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k)
+    {
+        uint64_t i = waveBase + uint64_t(k) * 64 + lane;
+        if (i < n)
+        {
+            unsigned d   = unsigned(key[k] >> shift) & 255u;
+            uint32_t pos = dStart[d] + wcount[wave][d] + rank[k];
+            sk[pos]      = key[k];
+            if constexpr (HasVals) sv[pos] = val[k];
+        }
+    }
+    __syncthreads();
+
+    uint64_t tileN = (n - tileBase) < uint64_t(SORT_TILE) ? (n - tileBase) : uint64_t(SORT_TILE);
+    for (unsigned i = threadIdx.x; i < tileN; i += SORT_THREADS)
+    {
+        uint64_t k   = sk[i];
+        unsigned d   = unsigned(k >> shift) & 255u;
+        uint64_t out = uint64_t(gOfs[d]) + (i - dStart[d]);
+        kout[out]    = k;
+        if constexpr (HasVals) vout[out] = sv[i];
+    }
+}
+
+// one pass over the keys: which digit passes are not constant
+__global__ __launch_bounds__(256) void radixDigitVariation(const uint64_t* __restrict__ keys, uint64_t n,
+                                                           uint64_t* __restrict__ orDiff)
+{
+    // OR over (key ^ key0): bit b set iff bit b varies
+    uint64_t k0  = keys[0];
+    uint64_t acc = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        acc |= keys[i] ^ k0;
+    for (int o = 32; o > 0; o >>= 1)
+        acc |= __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicOr((unsigned long long*)orDiff, (unsigned long long)acc);
+}
+
+__global__ void copyKV(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, uint64_t n)
+{
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    {
+        kd[i] = ks[i];
+        if (vs) vd[i] = vs[i];
+    }
+}
+
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits)
+{
+    if (n <= 1) return SPHX_OK;
+    if (n >= (uint64_t(1) << 32)) return argError(ctx, "sort: n must be < 2^32");
+    uint32_t numTiles = uint32_t((n + SORT_TILE - 1) / SORT_TILE);
+
+    uint64_t* kAlt = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
+    uint32_t* vAlt = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
+    uint32_t* hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
+    uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
+    if (!kAlt || (vals && !vAlt) || !hist || !vary) return SPHX_ERR_NOMEM;
+
+    // find the digit passes that change anything (one read of the keys + one 8-byte D2H)
+    SPHX_HIP(ctx, hipMemsetAsync(vary, 0, sizeof(uint64_t), ctx->stream));
+    unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8);
+    radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, vary);
+    SPHX_LAUNCH_CHECK(ctx, "radixDigitVariation");
+    uint64_t varies = 0;
+    SPHX_HIP(ctx, hipMemcpyAsync(&varies, vary, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+    uint64_t* kSrc = keys;
+    uint32_t* vSrc = vals;
+    uint64_t* kDst = kAlt;
+    uint32_t* vDst = vAlt;
+    for (int shift = 0; shift < keyBits; shift += 8)
+    {
+        if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
+        radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles);
+        SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
+        SPHX_TRY(exclusiveScanU32(ctx, hist, hist, uint64_t(RADIX) * numTiles));
+        if (vals)
+            radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
+                                                                          numTiles);
+        else
+            radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
+                                                                           hist, numTiles);
+        SPHX_LAUNCH_CHECK(ctx, "radixScatter");
+        std::swap(kSrc, kDst);
+        std::swap(vSrc, vDst);
+    }
+    if (kSrc != keys)
+    {
+        unsigned g = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 16);
+        copyKV<<<g, 256, 0, ctx->stream>>>(kSrc, vSrc, keys, vals, n);
+        SPHX_LAUNCH_CHECK(ctx, "copyKV");
+    }
+    return SPHX_OK;
+}
+
+// ===========================================================================
+// gather / sequence
+template<class T>
+__global__ __launch_bounds__(256) void gatherKernel(const uint32_t* __restrict__ order, const T* __restrict__ in,
+                                                    T* __restrict__ out, uint64_t n)
+{
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[order[i]];
+}
+
+__global__ __launch_bounds__(256) void sequenceKernel(uint32_t* v, uint64_t n, uint32_t first)
+{
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = first + uint32_t(i);
+}
+
+// ===========================================================================
+// Hilbert keys: sfc3D (domain/include/cstone/sfc/sfc.hpp:141-178) with separately
+// rounded products, as in the reference expression int(floor(x*mx) - xmin*mx)
+__global__ __launch_bounds__(256) void sfcKeysKernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, uint64_t* __restrict__ keys,
+                                                     uint64_t n, DBox box)
+{
+#pragma clang fp contract(off)
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (keys[i] == kRemoveKey) return;
+    const double cube   = double(1u << kMaxLevel);
+    const int    mcoord = (1 << kMaxLevel) - 1;
+    double       mx = cube * box.ilen[0], my = cube * box.ilen[1], mz = cube * box.ilen[2];
+    int          ix = int(floor(x[i] * mx) - box.lo[0] * mx);
+    int          iy = int(floor(y[i] * my) - box.lo[1] * my);
+    int          iz = int(floor(z[i] * mz) - box.lo[2] * mz);
+    ix = min(ix, mcoord);
+    iy = min(iy, mcoord);
+    iz = min(iz, mcoord);
+    ix = max(ix, 0); // coordinates below the box floor would be undefined in the reference
+    iy = max(iy, 0);
+    iz = max(iz, 0);
+    keys[i] = hilbertKey(unsigned(ix), unsigned(iy), unsigned(iz));
+}
+
+template<class T>
+int gatherT(sphx_ctx* ctx, const uint32_t* order, const T* in, T* out, uint64_t n)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (n == 0) return SPHX_OK;
+    if (!order || !in || !out) return argError(ctx, "gather: null pointer");
+    if (static_cast<const void*>(in) == static_cast<const void*>(out)) return argError(ctx, "gather: in == out");
+    gatherKernel<T><<<gridFor(n, 256), 256, 0, ctx->stream>>>(order, in, out, n);
+    SPHX_LAUNCH_CHECK(ctx, "gatherKernel");
+    return SPHX_OK;
+}
+
+} // namespace sphx
+
+using namespace sphx;
+
+// ===========================================================================
+// C ABI
+extern "C"
+{
+
+const char* sphx_version(void) { return "sphx 0.1 gfx950"; }
+
+const char* sphx_last_error(const sphx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sphx_ctx_create(int device, void* stream, sphx_ctx** out)
+{
+    if (!out) return SPHX_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    {
+        (void)hipGetLastError();
+        return SPHX_ERR_NODEV;
+    }
+    if (device < 0 || device >= n) return SPHX_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SPHX_ERR_HIP;
+    auto* ctx   = new sphx_ctx;
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    int cu      = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+        ctx->numCU = cu;
+    *out = ctx;
+    return SPHX_OK;
+}
+
+int sphx_ctx_destroy(sphx_ctx* ctx)
+{
+    if (!ctx) return SPHX_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->buf)
+        if (b.p) (void)hipFree(b.p);
+    delete ctx;
+    return SPHX_OK;
+}
+
+int sphx_ctx_set_stream(sphx_ctx* ctx, void* stream)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    return SPHX_OK;
+}
+
+int sphx_ctx_synchronize(sphx_ctx* ctx)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SPHX_OK;
+}
+
+int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const double* z, uint64_t* keys,
+                          uint64_t n, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (n == 0) return SPHX_OK;
+    if (!x || !y || !z || !keys) return argError(ctx, "compute_sfc_keys: null pointer");
+    if (!validBox(box)) return argError(ctx, "compute_sfc_keys: invalid box");
+    sfcKeysKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(x, y, z, keys, n, makeDBox(box));
+    SPHX_LAUNCH_CHECK(ctx, "sfcKeysKernel");
+    return SPHX_OK;
+}
+
+int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (n == 0) return SPHX_OK;
+    if (!keys || !values) return argError(ctx, "sort_by_key: null pointer");
+    return radixSortPairs(ctx, keys, values, n, 64);
+}
+
+int sphx_sequence(sphx_ctx* ctx, uint32_t* values, uint64_t n, uint32_t first)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (n == 0) return SPHX_OK;
+    if (!values) return argError(ctx, "sequence: null pointer");
+    sequenceKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(values, n, first);
+    SPHX_LAUNCH_CHECK(ctx, "sequenceKernel");
+    return SPHX_OK;
+}
+
+int sphx_gather_f64(sphx_ctx* ctx, const uint32_t* order, const double* in, double* out, uint64_t n)
+{
+    return gatherT(ctx, order, in, out, n);
+}
+int sphx_gather_f32(sphx_ctx* ctx, const uint32_t* order, const float* in, float* out, uint64_t n)
+{
+    return gatherT(ctx, order, in, out, n);
+}
+int sphx_gather_u32(sphx_ctx* ctx, const uint32_t* order, const uint32_t* in, uint32_t* out, uint64_t n)
+{
+    return gatherT(ctx, order, in, out, n);
+}
+
+} // extern "C"

@@ -1,0 +1,227 @@
+// sphx_hydro.hip - C ABI of the std-SPH loops (density, EOS, IAD,
+// momentum-energy), the ideal-gas EOS kernel and the host-side kernel tables.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sphx_internal.hpp"
+
+#define SPHX_HYDRO_NS fast
+#define SPHX_HYDRO_EXACT 0
+#include "sphx_hydro_kernels.inc"
+
+namespace sphx
+{
+namespace exact
+{
+int launchDensity(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const DBox&);
+int launchPrep(sphx_ctx*, uint32_t, uint32_t, sphx_particles*, void*, void*, void*);
+int launchIAD(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const void*, const DBox&);
+int launchMomentumEnergy(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const void*, const void*,
+                         const void*, const DBox&, unsigned long long*);
+} // namespace exact
+
+int groupsFromArgs(sphx_ctx* ctx, const sphx_groups* groups, uint32_t first, uint32_t last, GroupsDev& G);
+
+// idealGasEOS_u (sph/include/sph/eos.hpp:34-44) evaluated in double like the
+// reference's common_type with a double gamma
+template<class T>
+__global__ __launch_bounds__(256) void eosKernel(uint32_t first, uint32_t last, const T* __restrict__ u,
+                                                 const T* __restrict__ rho, double gamma, T* __restrict__ p,
+                                                 T* __restrict__ c)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) return;
+    double tmp = double(u[i]) * (gamma - 1.0);
+    p[i]       = T(double(rho[i]) * tmp);
+    c[i]       = T(sqrt(gamma * tmp));
+}
+
+__global__ void initMinDt(unsigned long long* v)
+{
+    if (threadIdx.x == 0) *v = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+static bool useExact(const sphx_ctx* ctx, const sphx_particles* d)
+{
+    return ctx->fp64Exact && d->precision == SPHX_FP64;
+}
+
+// the SPH loops run on the lists of the last sphx_find_neighbors_sph call and
+// require the same groups
+static int listsFor(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d, GroupsDev& G,
+                    const uint32_t*& lists)
+{
+    if (!ctx->nsValid) return fail(ctx, SPHX_ERR_STATE, "no neighbor lists: call sphx_find_neighbors_sph first");
+    SPHX_TRY(groupsFromArgs(ctx, groups, 0, uint32_t(d->n), G));
+    const GroupsDev& N = ctx->nsGroups;
+    if (G.first != N.first || G.last != N.last || G.numGroups != N.numGroups || G.start != N.start ||
+        G.end != N.end || d->ngmax != ctx->nsNgmax || d->nc != ctx->nsNc)
+        return fail(ctx, SPHX_ERR_STATE, "groups, ngmax or nc differ from the neighbor search");
+    lists = static_cast<const uint32_t*>(ctx->buf[S_NS_LISTS].p);
+    return SPHX_OK;
+}
+
+static int checkParticles(sphx_ctx* ctx, const sphx_particles* d, const char* what)
+{
+    if (!d) return argError(ctx, what);
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED) return argError(ctx, "unknown precision");
+    if (!d->x || !d->y || !d->z || !d->h || !d->m || !d->nc || !d->wh) return argError(ctx, what);
+    return SPHX_OK;
+}
+
+static size_t elemSize(const sphx_particles* d) { return d->precision == SPHX_FP64 ? 8 : 4; }
+
+} // namespace sphx
+
+using namespace sphx;
+
+extern "C"
+{
+
+int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (option == SPHX_OPT_FP64_EXACT)
+    {
+        ctx->fp64Exact = value ? 1 : 0;
+        return SPHX_OK;
+    }
+    return argError(ctx, "ctx_set_option: unknown option");
+}
+
+int sphx_kernel_tables(double sincIndex, double* wh64, float* wh32, double* K)
+{
+#pragma clang fp contract(off)
+    auto W = [sincIndex](double v) {
+        double s = (v == 0.0) ? 1.0 : std::sin(M_PI_2 * v) / (M_PI_2 * v); // wharmonic_std, kernels.hpp:36-43
+        return std::pow(s, sincIndex);
+    };
+    if (K)
+    {
+        // kernel_3D_k: 1 / Simpson(4 pi x^2 W(x), [0, 2], 2000 intervals) with sorted partial sums
+        const uint64_t n = 2000;
+        const double   a = 0.0, b = 2.0, hs = (b - a) / double(n);
+        auto f = [&](double x) { return 4.0 * M_PI * x * x * W(x); };
+        std::vector<double> odd(n / 2), even(n / 2 - 1);
+        for (uint64_t i = 0; i < odd.size(); ++i)
+            odd[i] = f(a + double(2 * (i + 1) - 1) * hs);
+        for (uint64_t i = 0; i < even.size(); ++i)
+            even[i] = f(a + double(2 * (i + 1)) * hs);
+        std::sort(odd.begin(), odd.end());
+        std::sort(even.begin(), even.end());
+        double so = 0.0, se = 0.0;
+        for (double v : odd)
+            so += v;
+        for (double v : even)
+            se += v;
+        *K = 1.0 / (hs / 3.0 * (f(a) + f(b) + 4.0 * so + 2.0 * se));
+    }
+    const int numIntervals = SPHX_TABLE_SIZE - 1;
+    if (wh64)
+    {
+        const double dx = 2.0 / numIntervals;
+        for (int i = 0; i < SPHX_TABLE_SIZE; ++i)
+            wh64[i] = W(0.0 + double(size_t(i)) * dx);
+    }
+    if (wh32)
+    {
+        const float dx = float(2.0 / numIntervals);
+        for (int i = 0; i < SPHX_TABLE_SIZE; ++i)
+        {
+            float v = float(0.0 + double(float(size_t(i)) * dx));
+            wh32[i] = float(W(double(v)));
+        }
+    }
+    return SPHX_OK;
+}
+
+int sphx_compute_density(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_density: null field"));
+    if (!d->rho) return argError(ctx, "compute_density: null rho");
+    if (!validBox(box)) return argError(ctx, "compute_density: invalid box");
+    GroupsDev       G;
+    const uint32_t* lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    DBox db = makeDBox(box);
+    return useExact(ctx, d) ? exact::launchDensity(ctx, G, lists, d, db) : fast::launchDensity(ctx, G, lists, d, db);
+}
+
+int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!d || !d->u || !d->rho || !d->p || !d->c) return argError(ctx, "compute_eos: null field");
+    if (last < first || last > d->n) return argError(ctx, "compute_eos: bad range");
+    if (last == first) return SPHX_OK;
+    if (d->precision == SPHX_FP64)
+        eosKernel<double><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
+            first, last, static_cast<const double*>(d->u), static_cast<const double*>(d->rho), d->gamma,
+            static_cast<double*>(d->p), static_cast<double*>(d->c));
+    else if (d->precision == SPHX_MIXED)
+        eosKernel<float><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
+            first, last, static_cast<const float*>(d->u), static_cast<const float*>(d->rho), d->gamma,
+            static_cast<float*>(d->p), static_cast<float*>(d->c));
+    else
+        return argError(ctx, "compute_eos: unknown precision");
+    SPHX_LAUNCH_CHECK(ctx, "eosKernel");
+    return SPHX_OK;
+}
+
+int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_iad: null field"));
+    if (!d->rho || !d->c11 || !d->c12 || !d->c13 || !d->c22 || !d->c23 || !d->c33)
+        return argError(ctx, "compute_iad: null field");
+    if (!validBox(box)) return argError(ctx, "compute_iad: invalid box");
+    GroupsDev       G;
+    const uint32_t* lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    void* vol = scratch(ctx, S_HY_VOL, elemSize(d) * d->n + 256);
+    if (!vol) return SPHX_ERR_NOMEM;
+    DBox db = makeDBox(box);
+    // m_j / rho_j is needed for every neighbor j, which may lie outside [first, last)
+    SPHX_TRY(fast::launchPrep(ctx, 0, uint32_t(d->n), d, nullptr, vol, nullptr));
+    return useExact(ctx, d) ? exact::launchIAD(ctx, G, lists, d, vol, db) : fast::launchIAD(ctx, G, lists, d, vol, db);
+}
+
+int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_momentum_energy: null field"));
+    if (!d->vx || !d->vy || !d->vz || !d->rho || !d->p || !d->c || !d->c11 || !d->c12 || !d->c13 || !d->c22 ||
+        !d->c23 || !d->c33 || !d->ax || !d->ay || !d->az || !d->du)
+        return argError(ctx, "compute_momentum_energy: null field");
+    if (!validBox(box)) return argError(ctx, "compute_momentum_energy: invalid box");
+    GroupsDev       G;
+    const uint32_t* lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    d->minDtCourant = INFINITY;
+    if (G.numGroups == 0) return SPHX_OK;
+    size_t es   = elemSize(d) * d->n + 256;
+    void*  vol  = scratch(ctx, S_HY_VOL, es);
+    void*  hinv = scratch(ctx, S_HY_HINV, es);
+    void*  prho = scratch(ctx, S_HY_PRHO, es);
+    auto*  dt   = scratchT<unsigned long long>(ctx, S_SMALL, 4);
+    if (!vol || !hinv || !prho || !dt) return SPHX_ERR_NOMEM;
+    DBox db = makeDBox(box);
+    SPHX_TRY(fast::launchPrep(ctx, 0, uint32_t(d->n), d, hinv, vol, prho));
+    initMinDt<<<1, 64, 0, ctx->stream>>>(dt);
+    SPHX_LAUNCH_CHECK(ctx, "initMinDt");
+    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, hinv, vol, prho, db, dt)
+                              : fast::launchMomentumEnergy(ctx, G, lists, d, hinv, vol, prho, db, dt));
+    unsigned long long bits = 0;
+    SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    d->minDtCourant = v;
+    return SPHX_OK;
+}
+
+} // extern "C"

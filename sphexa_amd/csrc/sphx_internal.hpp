@@ -1,0 +1,302 @@
+// sphx_internal.hpp - shared context, scratch arena and device helpers of libsphx.
+//
+// Target: gfx950 (CDNA4), wave64.  All kernels are launched on ctx->stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/sphx.h"
+
+namespace sphx
+{
+
+// ---------------------------------------------------------------------------
+// scratch arena: one grow-only device buffer per slot
+enum Slot : int
+{
+    S_SORT_KEYS = 0, // radix sort ping-pong keys
+    S_SORT_VALS,     // radix sort ping-pong values
+    S_SORT_HIST,     // per-tile digit histograms (+ their scan)
+    S_SCAN0,         // device-wide scan block sums, level 0
+    S_SCAN1,         // level 1
+    S_SCAN2,         // level 2
+    S_CS_NUMS,       // cornerstone: split-node count per particle (+ scan)
+    S_CS_CHILD,      // cornerstone: emitted child keys -> sorted leaves
+    S_CS_CHILDV,     // cornerstone: dummy sort values
+    S_CS_COUNTS,     // cornerstone: leaf counts
+    S_CS_LAYOUT,     // cornerstone: leaf layout
+    S_NS_RANGES,     // neighbor search: candidate particle ranges per group
+    S_NS_RCOUNT,     // neighbor search: range count per group
+    S_NS_LISTS,      // neighbor lists [group][k][lane]
+    S_NS_STATE,      // per-particle h-iteration counters / active flags
+    S_NS_REDO,       // groups that need another h iteration
+    S_GROUPS,        // computed group boundaries
+    S_GROUPS_TMP,    // group split scratch
+    S_SMALL,         // small device scalars (flags, counters, reductions)
+    S_TABLE64,       // sinc table, FP64
+    S_TABLE32,       // sinc table, FP32
+    S_HY_VOL,        // m / rho per particle
+    S_HY_HINV,       // 1 / h per particle
+    S_HY_PRHO,       // p / rho per particle
+    S_NUM_SLOTS
+};
+
+struct DevBuf
+{
+    void*  p     = nullptr;
+    size_t bytes = 0;
+};
+
+// target groups on the device: explicit [start, end) arrays or fixed 64-particle groups
+struct GroupsDev
+{
+    uint32_t        first = 0, last = 0, numGroups = 0;
+    const uint32_t* start = nullptr;
+    const uint32_t* end   = nullptr;
+};
+
+__host__ __device__ inline void groupRange(const GroupsDev& G, uint32_t g, uint32_t& s, uint32_t& e)
+{
+    if (G.start)
+    {
+        s = G.start[g];
+        e = G.end[g];
+    }
+    else
+    {
+        s = G.first + 64u * g;
+        e = (s + 64u < G.last) ? s + 64u : G.last;
+    }
+}
+
+// neighbor-list layout: GPU [group][k][lane] or the reference CPU layout [i - first][k]
+struct ListLayout
+{
+    uint32_t strideK; // distance between consecutive entries of one particle
+    uint32_t cpu;     // 1: [i - first][k]
+};
+
+__host__ __device__ inline size_t listBase(const ListLayout& lay, const GroupsDev& G, uint32_t g, uint32_t s,
+                                           uint32_t lane, uint32_t ngmax)
+{
+    return lay.cpu ? size_t(s + lane - G.first) * ngmax : size_t(g) * ngmax * 64 + lane;
+}
+
+} // namespace sphx
+
+struct sphx_ctx
+{
+    int             device = 0;
+    hipStream_t     stream = nullptr;
+    int             numCU  = 256;
+    std::string     err;
+    sphx::DevBuf    buf[sphx::S_NUM_SLOTS];
+    // neighbor-list bookkeeping: the SPH loops require lists of the same particle set
+    sphx::GroupsDev nsGroups;
+    uint32_t        nsNgmax = 0;
+    const uint32_t* nsNc    = nullptr;
+    bool            nsValid = false;
+    int             fp64Exact = 0; // 1: FP64 loops without FMA contraction (bit-exact vs the CPU path)
+    // cornerstone results (context-owned)
+    int32_t csNumLeaves = 0;
+    // computed groups
+    uint32_t grpNumGroups = 0;
+};
+
+namespace sphx
+{
+
+int fail(sphx_ctx* ctx, int code, const std::string& msg);
+int failHip(sphx_ctx* ctx, hipError_t e, const char* what);
+int argError(sphx_ctx* ctx, const char* what);
+
+// grow-only scratch; returns nullptr (and sets the error) on allocation failure
+void* scratch(sphx_ctx* ctx, Slot s, size_t bytes);
+
+template<class T>
+T* scratchT(sphx_ctx* ctx, Slot s, size_t count)
+{
+    return static_cast<T*>(scratch(ctx, s, count * sizeof(T) + 256));
+}
+
+// launch check after a kernel launch
+#define SPHX_LAUNCH_CHECK(ctx, name)                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                                             \
+        if (e_ != hipSuccess) return ::sphx::failHip(ctx, e_, name);                                                   \
+    } while (0)
+
+#define SPHX_HIP(ctx, call)                                                                                            \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) return ::sphx::failHip(ctx, e_, #call);                                                  \
+    } while (0)
+
+#define SPHX_TRY(expr)                                                                                                 \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        int s_ = (expr);                                                                                               \
+        if (s_ != SPHX_OK) return s_;                                                                                  \
+    } while (0)
+
+inline unsigned gridFor(uint64_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------
+// device-wide primitives (sphx_primitives.hip)
+
+// exclusive scan of n uint32 values into out[0..n] (out[n] = total); in may equal out
+int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n);
+
+// stable LSD radix sort of 64-bit keys (+ optional 32-bit values) in place;
+// only the bits in [0, keyBits) are examined
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits);
+
+// ---------------------------------------------------------------------------
+// key bit algebra (64-bit Hilbert keys, 21 levels, 63 key bits)
+constexpr unsigned kMaxLevel  = 21;
+constexpr uint64_t kRootRange = uint64_t(1) << 63;
+constexpr uint64_t kRemoveKey = uint64_t(1) << 63;
+
+__host__ __device__ inline unsigned clz64(uint64_t k)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return k == 0 ? 64u : unsigned(__clzll(k));
+#else
+    return k == 0 ? 64u : unsigned(__builtin_clzll(k));
+#endif
+}
+__host__ __device__ inline uint64_t nodeRange(unsigned level) { return uint64_t(1) << (3u * (kMaxLevel - level)); }
+__host__ __device__ inline unsigned treeLevel(uint64_t range) { return (clz64(range - 1) - 1) / 3; }
+__host__ __device__ inline int commonPrefix(uint64_t a, uint64_t b) { return int(clz64(a ^ b)) - 1; }
+__host__ __device__ inline uint64_t encodePlaceholderBit(uint64_t code, int prefixLength)
+{
+    return (uint64_t(1) << prefixLength) | (code >> (3 * int(kMaxLevel) - prefixLength));
+}
+__host__ __device__ inline unsigned decodePrefixLength(uint64_t code) { return 63u - clz64(code); }
+__host__ __device__ inline uint64_t decodePlaceholderBit(uint64_t code)
+{
+    unsigned len = decodePrefixLength(code);
+    return (code ^ (uint64_t(1) << len)) << (3 * kMaxLevel - len);
+}
+__host__ __device__ inline unsigned octalDigit(uint64_t code, unsigned position)
+{
+    return unsigned(code >> (3u * (kMaxLevel - position))) & 7u;
+}
+
+// Hilbert encode: octant order 0,1,3,2,7,6,4,5 packed as 3-bit fields
+__host__ __device__ inline uint64_t hilbertKey(unsigned px, unsigned py, unsigned pz)
+{
+    constexpr uint32_t m2h = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) | (4u << 18) |
+                             (5u << 21);
+    uint64_t key = 0;
+    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
+    {
+        unsigned xi = (px >> level) & 1u;
+        unsigned yi = (py >> level) & 1u;
+        unsigned zi = (pz >> level) & 1u;
+        unsigned octant = (xi << 2) | (yi << 1) | zi;
+        key = (key << 3) + ((m2h >> (3u * octant)) & 7u);
+
+        px ^= 0u - (xi & ((!yi) | zi));
+        py ^= 0u - ((xi & (yi | zi)) | (yi & (!zi)));
+        pz ^= 0u - ((xi & (!yi) & (!zi)) | (yi & (!zi)));
+
+        unsigned tx = px, ty = py, tz = pz;
+        if (zi) { px = ty, py = tz, pz = tx; }       // cyclic rotation
+        else if (!yi) { px = tz, pz = tx; }          // swap x and z
+    }
+    return key;
+}
+
+__host__ __device__ inline void hilbertDecode(uint64_t key, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (unsigned level = 0; level < kMaxLevel; ++level)
+    {
+        unsigned octant = unsigned(key >> (3 * level)) & 7u;
+        unsigned xi = octant >> 2u, yi = (octant >> 1u) & 1u, zi = octant & 1u;
+        unsigned tx = px, ty = py, tz = pz;
+        if (yi ^ zi) { px = tz, pz = ty, py = tx; }
+        else if ((!xi & !yi & !zi) || (xi & yi & zi)) { px = tz, pz = tx; }
+        unsigned mask = (1u << level) - 1;
+        px ^= mask & (0u - (xi & (yi | zi)));
+        py ^= mask & (0u - ((xi & ((!yi) | (!zi))) | ((!xi) & yi & zi)));
+        pz ^= mask & (0u - ((xi & (!yi) & (!zi)) | (yi & zi)));
+        px |= (xi << level);
+        py |= ((xi ^ yi) << level);
+        pz |= ((yi ^ zi) << level);
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+// ---------------------------------------------------------------------------
+// box with the derived quantities of cstone::Box
+struct DBox
+{
+    double lo[3], hi[3], len[3], ilen[3];
+    int    pbc[3]; // 1 if periodic
+    int    anyPbc;
+};
+
+inline DBox makeDBox(const sphx_box* b)
+{
+    DBox r;
+    r.lo[0] = b->xmin, r.hi[0] = b->xmax;
+    r.lo[1] = b->ymin, r.hi[1] = b->ymax;
+    r.lo[2] = b->zmin, r.hi[2] = b->zmax;
+    int bc[3] = {b->bx, b->by, b->bz};
+    r.anyPbc = 0;
+    for (int d = 0; d < 3; ++d)
+    {
+        r.len[d]  = r.hi[d] - r.lo[d];
+        r.ilen[d] = 1.0 / (r.hi[d] - r.lo[d]);
+        r.pbc[d]  = (bc[d] == SPHX_BC_PERIODIC);
+        r.anyPbc |= r.pbc[d];
+    }
+    return r;
+}
+
+bool validBox(const sphx_box* b);
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+__device__ inline unsigned laneId() { return __lane_id(); }
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline unsigned popc64(uint64_t m) { return unsigned(__popcll(m)); }
+__device__ inline uint64_t lanemaskLt() { return (uint64_t(1) << laneId()) - 1; }
+
+template<class T>
+__device__ inline T waveMin(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        T w = __shfl_xor(v, o, 64);
+        v   = w < v ? w : v;
+    }
+    return v;
+}
+template<class T>
+__device__ inline T waveMax(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        T w = __shfl_xor(v, o, 64);
+        v   = w > v ? w : v;
+    }
+    return v;
+}
+template<class T>
+__device__ inline T waveSum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+} // namespace sphx

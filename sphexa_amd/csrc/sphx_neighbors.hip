@@ -1,0 +1,531 @@
+// sphx_neighbors.hip - wave64 target-group neighbor search.
+//
+// One wave owns one target group (<= 64 SFC-consecutive particles, one per lane).
+//   nsGroupRanges : breadth-first walk of the octree with the group's search box
+//                   (union of the targets' 2h spheres, minimum image in periodic
+//                   dimensions); overlapping leaves are collected in LDS, sorted
+//                   (leaf order = SFC order = particle order) and merged into
+//                   contiguous particle ranges.
+//   nsCandidates  : the ranges are streamed through LDS 64 particles at a time;
+//                   every lane tests every staged candidate against its own target
+//                   with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
+//                   separately rounded operations (findneighbors.hpp:33-60,115-140),
+//                   so the neighbor sets are bit-identical to the CPU path; hits are
+//                   appended in ascending j, which is also the CPU's depth-first
+//                   order, so truncation at ngmax keeps the same entries.
+// The smoothing-length iteration of findNeighborsSph (sph/find_neighbors.hpp:10-41)
+// re-runs only the groups that still have particles outside [ng0/4, ngmax + 1].
+#include <algorithm>
+#include <climits>
+
+#include "sphx_internal.hpp"
+
+namespace sphx
+{
+
+constexpr int NS_FCAP = 2048; // BFS frontier capacity (per buffer)
+constexpr int NS_LCAP = 1024; // candidate leaves per group
+
+struct NsFlags
+{
+    uint32_t redoCount;   // groups appended to the redo list
+    uint32_t overflow;    // traversal capacity exceeded
+    uint32_t badGroup;    // a group with more than 64 particles
+    uint32_t pad;
+};
+
+__device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
+                                       const double gc[3], const double gh[3], const DBox& box)
+{
+    bool ov = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        double dx = centers[3 * node + d] - gc[d];
+        if (box.pbc[d]) dx -= box.len[d] * rint(dx * box.ilen[d]);
+        ov = ov && (fabs(dx) <= sizes[3 * node + d] + gh[d]);
+    }
+    return ov;
+}
+
+template<class Th>
+__global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t* __restrict__ groupList,
+                                                    const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const Th* __restrict__ h,
+                                                    const uint8_t* __restrict__ active,
+                                                    const int32_t* __restrict__ childOffsets,
+                                                    const int32_t* __restrict__ internalToLeaf,
+                                                    const uint32_t* __restrict__ layout,
+                                                    const double* __restrict__ centers,
+                                                    const double* __restrict__ sizes, DBox box,
+                                                    uint2* __restrict__ ranges, uint32_t* __restrict__ rangeCount,
+                                                    NsFlags* __restrict__ flags)
+{
+    __shared__ int32_t front[2][NS_FCAP];
+    __shared__ int32_t leafList[NS_LCAP];
+
+    const uint32_t g    = groupList ? groupList[blockIdx.x] : blockIdx.x;
+    const unsigned lane = threadIdx.x;
+    uint32_t       s, e;
+    groupRange(G, g, s, e);
+    if (e - s > 64)
+    {
+        if (lane == 0) atomicOr(&flags->badGroup, 1u);
+        e = s + 64;
+    }
+    uint32_t i   = s + lane;
+    bool     act = (i < e) && (!active || active[i]);
+
+    double lo[3], hi[3];
+    {
+        double px[3] = {0, 0, 0};
+        double r     = 0;
+        if (act)
+        {
+            px[0] = x[i], px[1] = y[i], px[2] = z[i];
+            r     = 2.0 * double(h[i]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            lo[d] = waveMin(act ? px[d] - r : INFINITY);
+            hi[d] = waveMax(act ? px[d] + r : -INFINITY);
+        }
+    }
+    if (!(lo[0] <= hi[0]))
+    {
+        if (lane == 0) rangeCount[g] = 0;
+        return;
+    }
+    double gc[3], gh[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        gc[d] = 0.5 * (lo[d] + hi[d]);
+        gh[d] = 0.5 * (hi[d] - lo[d]);
+        gh[d] += 1e-12 * (fabs(gc[d]) + gh[d]); // never under-cover the union of spheres
+    }
+
+    // ---- breadth-first traversal
+    int  lcount   = 0;
+    bool overflow = false;
+    int  fcount   = 0;
+    int  cur      = 0;
+    if (childOffsets[0] == 0)
+    {
+        if (lane == 0) leafList[0] = internalToLeaf[0];
+        lcount = 1;
+    }
+    else
+    {
+        if (lane < 8) front[0][lane] = childOffsets[0] + int(lane);
+        fcount = 8;
+    }
+    __syncthreads();
+    const uint64_t lt = lanemaskLt();
+    while (fcount > 0 && !overflow)
+    {
+        int ncount = 0;
+        int nxt    = cur ^ 1;
+        for (int b = 0; b < fcount; b += 64)
+        {
+            int  idx   = b + int(lane);
+            int  node  = idx < fcount ? front[cur][idx] : -1;
+            bool ov    = node >= 0 && nodeOverlapsBox(centers, sizes, node, gc, gh, box);
+            int  child = ov ? childOffsets[node] : 0;
+            bool leaf  = ov && child == 0;
+            bool inner = ov && child != 0;
+
+            uint64_t mL   = ballot(leaf);
+            int      posL = lcount + int(popc64(mL & lt));
+            if (leaf && posL < NS_LCAP) leafList[posL] = internalToLeaf[node];
+            lcount += int(popc64(mL));
+
+            uint64_t mI   = ballot(inner);
+            int      posI = ncount + 8 * int(popc64(mI & lt));
+            if (inner && posI + 8 <= NS_FCAP)
+            {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    front[nxt][posI + c] = child + c;
+            }
+            ncount += 8 * int(popc64(mI));
+        }
+        __syncthreads();
+        if (lcount > NS_LCAP || ncount > NS_FCAP) overflow = true;
+        cur    = nxt;
+        fcount = ncount;
+    }
+    if (overflow)
+    {
+        if (lane == 0)
+        {
+            atomicOr(&flags->overflow, 1u);
+            rangeCount[g] = 0;
+        }
+        return;
+    }
+
+    // ---- bitonic sort of the candidate leaves (ascending)
+    int P = 1;
+    while (P < lcount)
+        P <<= 1;
+    for (int k = lcount + int(lane); k < P; k += 64)
+        leafList[k] = INT_MAX;
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+    {
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+        {
+            for (int t = int(lane); t < P / 2; t += 64)
+            {
+                int  a   = 2 * t - (t & (stride - 1));
+                int  b2  = a + stride;
+                bool up  = ((a & size) == 0);
+                int  va  = leafList[a], vb = leafList[b2];
+                bool swp = up ? (va > vb) : (va < vb);
+                if (swp)
+                {
+                    leafList[a]  = vb;
+                    leafList[b2] = va;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- merge consecutive leaves into particle ranges
+    int rcount = 0;
+    for (int b = 0; b < lcount; b += 64)
+    {
+        int      k       = b + int(lane);
+        bool     inRange = k < lcount;
+        bool     isStart = inRange && (k == 0 || leafList[k] != leafList[k - 1] + 1);
+        uint64_t mS      = ballot(isStart);
+        int      r       = rcount + int(popc64(mS & lt));
+        if (isStart)
+        {
+            // find the end of this run
+            int kk = k;
+            while (kk + 1 < lcount && leafList[kk + 1] == leafList[kk] + 1)
+                ++kk;
+            uint32_t ps = layout[leafList[k]];
+            uint32_t pe = layout[leafList[kk] + 1];
+            ranges[size_t(g) * NS_LCAP + r] = make_uint2(ps, pe);
+        }
+        rcount += int(popc64(mS));
+    }
+    if (lane == 0) rangeCount[g] = uint32_t(rcount);
+}
+
+// updateH (sph/kernels.hpp:27-33)
+__device__ inline double updateH(unsigned ng0, unsigned nc, double h)
+{
+    return h * 0.5 * pow(1.0 + 1023.0 * double(ng0) / double(nc), 0.1);
+}
+
+// Candidate test.  Sph = true: findNeighborsSph semantics (nc counts self, h updates,
+// redo flags); Sph = false: findNeighbors semantics (count without self).
+template<class Th, bool Sph>
+__global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
+                                                   const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, Th* __restrict__ h,
+                                                   uint8_t* __restrict__ active, uint8_t* __restrict__ iter,
+                                                   const uint2* __restrict__ ranges,
+                                                   const uint32_t* __restrict__ rangeCount, DBox box, uint32_t ngmax,
+                                                   uint32_t ng0, ListLayout lay, uint32_t* __restrict__ nbrs,
+                                                   uint32_t* __restrict__ nc, uint32_t* __restrict__ redoList,
+                                                   NsFlags* __restrict__ flags)
+{
+#pragma clang fp contract(off)
+    __shared__ double sx[64], sy[64], sz[64];
+
+    const uint32_t g    = groupList ? groupList[blockIdx.x] : blockIdx.x;
+    const unsigned lane = threadIdx.x;
+    uint32_t       s, e;
+    groupRange(G, g, s, e);
+    if (e - s > 64) e = s + 64;
+    const uint32_t i   = s + lane;
+    const bool     act = (i < e) && (!active || active[i]);
+
+    double xi = 0, yi = 0, zi = 0, hi = 0;
+    if (act) xi = x[i], yi = y[i], zi = z[i], hi = double(h[i]);
+    const double radiusSq = 4.0 * hi * hi;
+    bool         inside   = true;
+    {
+        double p[3] = {xi, yi, zi};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            inside = inside && (p[d] - 2.0 * hi >= box.lo[d]) && (p[d] + 2.0 * hi <= box.hi[d]);
+    }
+    const bool usePbc = box.anyPbc && !inside;
+    const double px = box.pbc[0] ? box.len[0] : 0.0;
+    const double py = box.pbc[1] ? box.len[1] : 0.0;
+    const double pz = box.pbc[2] ? box.len[2] : 0.0;
+
+    size_t   lbase = listBase(lay, G, g, s, lane, ngmax);
+    uint32_t count = 0;
+
+    const uint32_t R = rangeCount[g];
+    for (uint32_t r = 0; r < R; ++r)
+    {
+        uint2 rg = ranges[size_t(g) * NS_LCAP + r];
+        for (uint32_t base = rg.x; base < rg.y; base += 64)
+        {
+            uint32_t m = min(64u, rg.y - base);
+            if (lane < m)
+            {
+                sx[lane] = x[base + lane];
+                sy[lane] = y[base + lane];
+                sz[lane] = z[base + lane];
+            }
+            __syncthreads();
+            if (act)
+            {
+                for (uint32_t k = 0; k < m; ++k)
+                {
+                    double dx = sx[k] - xi;
+                    double dy = sy[k] - yi;
+                    double dz = sz[k] - zi;
+                    if (usePbc)
+                    {
+                        dx -= px * rint(dx * box.ilen[0]);
+                        dy -= py * rint(dy * box.ilen[1]);
+                        dz -= pz * rint(dz * box.ilen[2]);
+                    }
+                    double   d2 = dx * dx + dy * dy + dz * dz;
+                    uint32_t j  = base + k;
+                    if (d2 < radiusSq && j != i)
+                    {
+                        if (nbrs && count < ngmax) nbrs[lbase + size_t(count) * lay.strideK] = j;
+                        ++count;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    if constexpr (Sph)
+    {
+        bool redo = false;
+        if (act)
+        {
+            uint32_t ncSph = count + 1;
+            uint32_t ngmin = ng0 / 4;
+            bool     out   = (ngmin > ncSph) || (ncSph - 1 > ngmax);
+            nc[i] = ncSph; // dataset counts are indexed by particle (sph/find_neighbors.hpp:51)
+            if (out && iter[i] < 10)
+            {
+                h[i] = Th(updateH(ng0, ncSph, hi));
+                iter[i] += 1;
+                redo = true;
+            }
+            active[i] = redo ? 1 : 0;
+        }
+        uint64_t anyRedo = ballot(redo);
+        if (anyRedo && lane == 0)
+        {
+            uint32_t pos  = atomicAdd(&flags->redoCount, 1u);
+            redoList[pos] = g;
+        }
+    }
+    else
+    {
+        if (act) nc[i - G.first] = count;
+    }
+}
+
+__global__ __launch_bounds__(256) void nsInitState(uint8_t* active, uint8_t* iter, uint32_t first, uint32_t last)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) return;
+    active[i] = 1;
+    iter[i]   = 0;
+}
+
+__global__ void nsResetFlags(NsFlags* f, int clearErrors)
+{
+    if (threadIdx.x == 0)
+    {
+        f->redoCount = 0;
+        if (clearErrors)
+        {
+            f->overflow = 0;
+            f->badGroup = 0;
+        }
+    }
+}
+
+// export [group][k][lane] lists into the reference CPU layout [i - first][k]
+__global__ __launch_bounds__(256) void nsExport(GroupsDev G, const uint32_t* __restrict__ lists, uint32_t ngmax,
+                                                const uint32_t* __restrict__ nc, uint32_t* __restrict__ out)
+{
+    uint32_t g    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t lane = threadIdx.x & 63u;
+    if (g >= G.numGroups) return;
+    uint32_t s, e;
+    groupRange(G, g, s, e);
+    uint32_t i = s + lane;
+    if (i >= e || lane >= 64) return;
+    uint32_t cnt = nc[i] - 1u;
+    cnt          = min(cnt, ngmax);
+    size_t base  = size_t(g) * ngmax * 64 + lane;
+    for (uint32_t k = 0; k < cnt; ++k)
+        out[size_t(i - G.first) * ngmax + k] = lists[base + size_t(k) * 64];
+}
+
+// ---------------------------------------------------------------------------
+// host driver shared by both entry points
+template<class Th, bool Sph>
+static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const double* y, const double* z, Th* h,
+                     const sphx_box* box, const sphx_octree* t, uint32_t ngmax, uint32_t ng0, ListLayout lay,
+                     uint32_t* nbrs, uint32_t* nc)
+{
+    if (G.numGroups == 0) return SPHX_OK;
+    uint32_t numG   = G.numGroups;
+    uint2*   ranges = scratchT<uint2>(ctx, S_NS_RANGES, size_t(numG) * NS_LCAP);
+    uint32_t* rcnt  = scratchT<uint32_t>(ctx, S_NS_RCOUNT, numG);
+    uint8_t* state  = scratchT<uint8_t>(ctx, S_NS_STATE, 2 * size_t(G.last));
+    uint32_t* redo  = scratchT<uint32_t>(ctx, S_NS_REDO, 2 * size_t(numG));
+    NsFlags* flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
+    if (!ranges || !rcnt || !state || !redo || !flags) return SPHX_ERR_NOMEM;
+    uint8_t* active = state;
+    uint8_t* iter   = state + G.last;
+    DBox     db     = makeDBox(box);
+
+    if (G.last > G.first)
+    {
+        nsInitState<<<gridFor(G.last - G.first, 256), 256, 0, ctx->stream>>>(active, iter, G.first, G.last);
+        SPHX_LAUNCH_CHECK(ctx, "nsInitState");
+    }
+    nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 1);
+    SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
+
+    const uint32_t* list     = nullptr; // nullptr: all groups
+    uint32_t        numLaunch = numG;
+    int             pingpong = 0;
+    for (int pass = 0; pass < 12 && numLaunch > 0; ++pass)
+    {
+        uint32_t* outList = redo + size_t(pingpong) * numG;
+        nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, t->childOffsets,
+                                                            t->internalToLeaf, t->layout, t->centers, t->sizes, db,
+                                                            ranges, rcnt, flags);
+        SPHX_LAUNCH_CHECK(ctx, "nsGroupRanges");
+        nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, iter, ranges, rcnt,
+                                                                db, ngmax, ng0, lay, nbrs, nc, outList, flags);
+        SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
+        NsFlags hf;
+        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));
+        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
+        if (hf.badGroup) return argError(ctx, "neighbor search: a target group holds more than 64 particles");
+        if (!Sph) break;
+        numLaunch = hf.redoCount;
+        list      = outList;
+        pingpong ^= 1;
+        nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 0);
+        SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
+    }
+    return SPHX_OK;
+}
+
+int groupsFromArgs(sphx_ctx* ctx, const sphx_groups* groups, uint32_t first, uint32_t last, GroupsDev& G)
+{
+    if (groups && groups->groupStart)
+    {
+        if (!groups->groupEnd) return argError(ctx, "groups: groupEnd is null");
+        G.first     = groups->firstBody;
+        G.last      = groups->lastBody;
+        G.numGroups = groups->numGroups;
+        G.start     = groups->groupStart;
+        G.end       = groups->groupEnd;
+    }
+    else
+    {
+        G.first     = groups ? groups->firstBody : first;
+        G.last      = groups ? groups->lastBody : last;
+        G.numGroups = (G.last - G.first + 63) / 64;
+        G.start     = nullptr;
+        G.end       = nullptr;
+    }
+    if (G.last < G.first) return argError(ctx, "groups: lastBody < firstBody");
+    return SPHX_OK;
+}
+
+static bool validTree(const sphx_octree* t)
+{
+    return t && t->numNodes > 0 && t->childOffsets && t->internalToLeaf && t->layout && t->centers && t->sizes;
+}
+
+} // namespace sphx
+
+using namespace sphx;
+
+extern "C"
+{
+
+int sphx_find_neighbors(sphx_ctx* ctx, const double* x, const double* y, const double* z, const double* h,
+                        uint32_t first, uint32_t last, const sphx_box* box, const sphx_octree* tree, uint32_t ngmax,
+                        uint32_t* nbrs, uint32_t* nc)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (last < first) return argError(ctx, "find_neighbors: last < first");
+    if (last == first) return SPHX_OK;
+    if (!x || !y || !z || !h || !nc) return argError(ctx, "find_neighbors: null pointer");
+    if (!validBox(box)) return argError(ctx, "find_neighbors: invalid box");
+    if (!validTree(tree)) return argError(ctx, "find_neighbors: incomplete tree view");
+    GroupsDev G;
+    SPHX_TRY(groupsFromArgs(ctx, nullptr, first, last, G));
+    ListLayout lay{1, 1}; // CPU layout: [i - first][k]
+    return runSearch<double, false>(ctx, G, x, y, z, const_cast<double*>(h), box, tree, ngmax, 0, lay, nbrs, nc);
+}
+
+int sphx_find_neighbors_sph(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box,
+                            const sphx_octree* tree)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!d || !d->x || !d->y || !d->z || !d->h || !d->nc) return argError(ctx, "find_neighbors_sph: null field");
+    if (!validBox(box)) return argError(ctx, "find_neighbors_sph: invalid box");
+    if (!validTree(tree)) return argError(ctx, "find_neighbors_sph: incomplete tree view");
+    if (d->ng0 > d->ngmax) return fail(ctx, SPHX_ERR_NGMAX, "ng0 should be smaller than ngmax");
+    if (d->ngmax == 0) return argError(ctx, "find_neighbors_sph: ngmax must be > 0");
+    GroupsDev G;
+    SPHX_TRY(groupsFromArgs(ctx, groups, 0, uint32_t(d->n), G));
+    if (G.last > d->n) return argError(ctx, "find_neighbors_sph: group range exceeds n");
+    ctx->nsValid = false;
+    size_t    listLen = size_t(G.numGroups) * d->ngmax * 64;
+    uint32_t* lists   = scratchT<uint32_t>(ctx, S_NS_LISTS, listLen);
+    if (!lists) return SPHX_ERR_NOMEM;
+    ListLayout lay{64, 0}; // GPU layout: [group][k][lane]
+    int        st;
+    if (d->precision == SPHX_FP64)
+        st = runSearch<double, true>(ctx, G, d->x, d->y, d->z, static_cast<double*>(d->h), box, tree, d->ngmax,
+                                     d->ng0, lay, lists, d->nc);
+    else if (d->precision == SPHX_MIXED)
+        st = runSearch<float, true>(ctx, G, d->x, d->y, d->z, static_cast<float*>(d->h), box, tree, d->ngmax,
+                                    d->ng0, lay, lists, d->nc);
+    else
+        return argError(ctx, "find_neighbors_sph: unknown precision");
+    if (st != SPHX_OK) return st;
+    ctx->nsValid  = true;
+    ctx->nsGroups = G;
+    ctx->nsNgmax  = d->ngmax;
+    ctx->nsNc     = d->nc;
+    return SPHX_OK;
+}
+
+int sphx_export_neighbors(sphx_ctx* ctx, uint32_t* nbrs)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!ctx->nsValid) return fail(ctx, SPHX_ERR_STATE, "export_neighbors: no neighbor lists in context");
+    if (!nbrs) return argError(ctx, "export_neighbors: null pointer");
+    const GroupsDev& G = ctx->nsGroups;
+    if (G.numGroups == 0) return SPHX_OK;
+    const uint32_t* lists = static_cast<const uint32_t*>(ctx->buf[S_NS_LISTS].p);
+    nsExport<<<gridFor(G.numGroups, 4), 256, 0, ctx->stream>>>(G, lists, ctx->nsNgmax, ctx->nsNc, nbrs);
+    SPHX_LAUNCH_CHECK(ctx, "nsExport");
+    return SPHX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,416 @@
+"""Host side of the MI355X SPH hot path: a Python mirror of the reference's C++
+entry points over torch device tensors, calling libsphx.so through its C ABI.
+
+Names follow the reference interface:
+
+=============================  ==============================================================
+this module                    reference entry point
+=============================  ==============================================================
+``compute_sfc_keys``           cstone::computeSfcKeys<true> (domain/include/cstone/sfc/sfc_gpu.h:26)
+``sort_by_key``                cstone::sortByKey<true> (primitives/primitives_acc.hpp:115)
+``gather``                     cstone::gather (primitives/gather.hpp:90)
+``compute_octree``             cstone::computeOctree (tree/csarray.hpp:430)
+``build_octree``               cstone::updateInternalTree / buildOctreeGpu (tree/octree.hpp:361)
+``node_fp_centers``            cstone::nodeFpCenters (focus/source_center.hpp:146)
+``find_neighbors``             cstone::findNeighbors (findneighbors.hpp:149)
+``find_neighbors_sfc``         sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:44)
+``compute_density``            sph::computeDensity (sph/include/sph/hydro_std/density.hpp:55)
+``compute_eos_hydro_std``      sph::computeEOS_HydroStd (hydro_std/eos.hpp:144)
+``compute_iad``                sph::computeIAD (hydro_std/iad.hpp:74)
+``compute_momentum_energy_std``sph::computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:96)
+``HydroStep.compute_forces``   HydroProp::computeForces (main/src/propagator/std_hydro.hpp:119)
+=============================  ==============================================================
+
+Errors raise ``SphxError`` carrying the library's status code; there is no CPU
+fallback.  Device memory comes from torch; the library's context launches on
+torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import Box, Groups, Octree, Particles, SphxError, FP64, MIXED, BC_OPEN, BC_PERIODIC, BC_FIXED
+
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpyAsync.restype = C.c_int
+        _hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    return _hip
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def make_box(lo, hi, bc=BC_OPEN):
+    """cstone::Box<double>: lo/hi scalars or 3-sequences, bc scalar or 3-sequence."""
+    lo3 = [lo] * 3 if np.isscalar(lo) else list(lo)
+    hi3 = [hi] * 3 if np.isscalar(hi) else list(hi)
+    bc3 = [bc] * 3 if np.isscalar(bc) else list(bc)
+    return Box(lo3[0], hi3[0], lo3[1], hi3[1], lo3[2], hi3[2], bc3[0], bc3[1], bc3[2], 0)
+
+
+class Context:
+    """Owns a sphx_ctx bound to torch's current stream on ``device``."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("sphexa_amd needs a HIP device (torch.cuda.is_available() is False)")
+        self.device = torch.device("cuda", device)
+        self.stream = torch.cuda.current_stream(self.device)
+        h = C.c_void_p()
+        st = self.lib.sphx_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(h))
+        if st != 0:
+            raise SphxError(st, "sphx_ctx_create failed")
+        self.h = h
+
+    def check(self, st):
+        if st != 0:
+            raise SphxError(st, self.lib.sphx_last_error(self.h).decode())
+
+    def set_fp64_exact(self, on: bool):
+        self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_FP64_EXACT, int(bool(on))))
+
+    def synchronize(self):
+        self.check(self.lib.sphx_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.sphx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def d2d(self, dst: torch.Tensor, src_ptr: int, nbytes: int):
+        st = _hiprt().hipMemcpyAsync(C.c_void_p(dst.data_ptr()), C.c_void_p(src_ptr), nbytes, 3,
+                                     C.c_void_p(self.stream.cuda_stream))
+        if st != 0:
+            raise RuntimeError(f"hipMemcpyAsync failed ({st})")
+
+
+def kernel_tables(sinc_index: float = 6.0):
+    """(wh64 numpy, wh32 numpy, K): ParticlesData::createTables (particles_data.hpp:377-387)."""
+    lib = _lib.load()
+    wh64 = np.zeros(_lib.TABLE_SIZE, np.float64)
+    wh32 = np.zeros(_lib.TABLE_SIZE, np.float32)
+    K = C.c_double()
+    lib.sphx_kernel_tables(sinc_index, wh64.ctypes.data_as(C.c_void_p), wh32.ctypes.data_as(C.c_void_p),
+                           C.byref(K))
+    return wh64, wh32, K.value
+
+
+# ------------------------------------------------------------------------------------------- SFC, sort
+def compute_sfc_keys(ctx: Context, x, y, z, box: Box, keys=None):
+    n = x.numel()
+    if keys is None:
+        keys = torch.zeros(n, dtype=torch.int64, device=x.device)
+    ctx.check(ctx.lib.sphx_compute_sfc_keys(ctx.h, _ptr(x), _ptr(y), _ptr(z), _ptr(keys), n, C.byref(box)))
+    return keys
+
+
+def sequence(ctx: Context, n: int, first: int = 0, device=None):
+    v = torch.empty(n, dtype=torch.int32, device=device or ctx.device)
+    ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(v), n, first))
+    return v
+
+
+def sort_by_key(ctx: Context, keys, values):
+    """Stable in-place sort of (int64-stored uint64 keys, int32-stored uint32 values)."""
+    ctx.check(ctx.lib.sphx_sort_by_key(ctx.h, _ptr(keys), _ptr(values), keys.numel()))
+
+
+def gather(ctx: Context, order, src, dst=None):
+    dst = torch.empty_like(src) if dst is None else dst
+    fn = {torch.float64: ctx.lib.sphx_gather_f64, torch.float32: ctx.lib.sphx_gather_f32,
+          torch.int32: ctx.lib.sphx_gather_u32}[src.dtype]
+    ctx.check(fn(ctx.h, _ptr(order), _ptr(src), _ptr(dst), src.numel()))
+    return dst
+
+
+# ------------------------------------------------------------------------------------------- octree
+def compute_octree(ctx: Context, keys, bucket_size: int = 64, max_count: int = 0xFFFFFFFF):
+    """(leaves int64[L+1], counts int32[L], layout int32[L+1]) as new device tensors."""
+    L = C.c_int32()
+    pl, pc, py = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.sphx_compute_octree(ctx.h, _ptr(keys), keys.numel(), bucket_size, max_count, C.byref(L),
+                                          C.byref(pl), C.byref(pc), C.byref(py)))
+    n = L.value
+    leaves = torch.empty(n + 1, dtype=torch.int64, device=keys.device)
+    counts = torch.empty(n, dtype=torch.int32, device=keys.device)
+    layout = torch.empty(n + 1, dtype=torch.int32, device=keys.device)
+    ctx.d2d(leaves, pl.value, 8 * (n + 1))
+    ctx.d2d(counts, pc.value, 4 * n)
+    ctx.d2d(layout, py.value, 4 * (n + 1))
+    return leaves, counts, layout
+
+
+@dataclass
+class OctreeData:
+    """Device arrays of the internal octree + neighbor-search view (OctreeData, OctreeNsView)."""
+
+    numLeafNodes: int
+    numInternalNodes: int
+    numNodes: int
+    prefixes: torch.Tensor
+    childOffsets: torch.Tensor
+    parents: torch.Tensor
+    levelRange: torch.Tensor
+    internalToLeaf: torch.Tensor
+    leafToInternal: torch.Tensor
+    leaves: torch.Tensor
+    layout: torch.Tensor | None = None
+    centers: torch.Tensor | None = None
+    sizes: torch.Tensor | None = None
+
+    def view(self) -> Octree:
+        return Octree(self.numLeafNodes, self.numInternalNodes, self.numNodes, 0, _ptr(self.prefixes),
+                      _ptr(self.childOffsets), _ptr(self.parents), _ptr(self.levelRange), _ptr(self.internalToLeaf),
+                      _ptr(self.leafToInternal), _ptr(self.leaves), _ptr(self.layout), _ptr(self.centers),
+                      _ptr(self.sizes))
+
+
+def build_octree(ctx: Context, leaves):
+    L = leaves.numel() - 1
+    ni = (L - 1) // 7
+    nn = L + ni
+    dev = leaves.device
+    o = OctreeData(L, ni, nn, torch.empty(nn, dtype=torch.int64, device=dev),
+                   torch.empty(nn + 1, dtype=torch.int32, device=dev),
+                   torch.empty(max(1, (nn - 1) // 8), dtype=torch.int32, device=dev),
+                   torch.empty(23, dtype=torch.int32, device=dev), torch.empty(nn, dtype=torch.int32, device=dev),
+                   torch.empty(nn, dtype=torch.int32, device=dev), leaves)
+    v = o.view()
+    ctx.check(ctx.lib.sphx_build_octree(ctx.h, _ptr(leaves), C.byref(v)))
+    return o
+
+
+def node_fp_centers(ctx: Context, octree: OctreeData, box: Box):
+    nn = octree.numNodes
+    octree.centers = torch.empty(nn * 3, dtype=torch.float64, device=octree.prefixes.device)
+    octree.sizes = torch.empty(nn * 3, dtype=torch.float64, device=octree.prefixes.device)
+    ctx.check(ctx.lib.sphx_node_centers(ctx.h, _ptr(octree.prefixes), nn, C.byref(box), _ptr(octree.centers),
+                                        _ptr(octree.sizes)))
+    return octree.centers, octree.sizes
+
+
+def tree_from_keys(ctx: Context, keys, box: Box, bucket_size: int = 64):
+    """Single-rank harness: leaves, internal tree, layout, centres (findneighbors.cpp:59-83)."""
+    leaves, counts, layout = compute_octree(ctx, keys, bucket_size)
+    o = build_octree(ctx, leaves)
+    o.layout = layout
+    o.counts = counts
+    node_fp_centers(ctx, o, box)
+    return o
+
+
+# ------------------------------------------------------------------------------------------- neighbors
+def find_neighbors(ctx: Context, x, y, z, h, box: Box, octree: OctreeData, ngmax: int, first=0, last=None,
+                   lists=True):
+    """cstone::findNeighbors: (nbrs int32[n, ngmax] or None, nc int32[n]) for [first, last)."""
+    last = x.numel() if last is None else last
+    n = last - first
+    nc = torch.zeros(n, dtype=torch.int32, device=x.device)
+    nb = torch.zeros(n * ngmax, dtype=torch.int32, device=x.device) if lists else None
+    v = octree.view()
+    ctx.check(ctx.lib.sphx_find_neighbors(ctx.h, _ptr(x), _ptr(y), _ptr(z), _ptr(h), first, last, C.byref(box),
+                                          C.byref(v), ngmax, _ptr(nb), _ptr(nc)))
+    return (nb.view(n, ngmax) if nb is not None else None), nc
+
+
+@dataclass
+class ParticlesData:
+    """Device particle fields (sphexa::ParticlesData field contract, particles_data.hpp:218-246)."""
+
+    n: int
+    precision: int = FP64
+    ng0: int = 100
+    ngmax: int = 150
+    Kcour: float = 0.2
+    gamma: float = 5.0 / 3.0
+    sincIndex: float = 6.0
+    K: float = 0.0
+    x: torch.Tensor | None = None
+    y: torch.Tensor | None = None
+    z: torch.Tensor | None = None
+    h: torch.Tensor | None = None
+    m: torch.Tensor | None = None
+    vx: torch.Tensor | None = None
+    vy: torch.Tensor | None = None
+    vz: torch.Tensor | None = None
+    u: torch.Tensor | None = None
+    rho: torch.Tensor | None = None
+    p: torch.Tensor | None = None
+    c: torch.Tensor | None = None
+    c11: torch.Tensor | None = None
+    c12: torch.Tensor | None = None
+    c13: torch.Tensor | None = None
+    c22: torch.Tensor | None = None
+    c23: torch.Tensor | None = None
+    c33: torch.Tensor | None = None
+    ax: torch.Tensor | None = None
+    ay: torch.Tensor | None = None
+    az: torch.Tensor | None = None
+    du: torch.Tensor | None = None
+    nc: torch.Tensor | None = None
+    keys: torch.Tensor | None = None
+    wh: torch.Tensor | None = None
+    minDtCourant: float = math.inf
+
+    HYDRO = ("h", "m", "vx", "vy", "vz", "u", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33", "ax",
+             "ay", "az", "du")
+
+    @property
+    def dtype(self):
+        return torch.float64 if self.precision == FP64 else torch.float32
+
+    def allocate(self, device):
+        """Allocate every missing field, make the kernel table and K."""
+        for f in ("x", "y", "z"):
+            if getattr(self, f) is None:
+                setattr(self, f, torch.zeros(self.n, dtype=torch.float64, device=device))
+        for f in self.HYDRO:
+            if getattr(self, f) is None:
+                setattr(self, f, torch.zeros(self.n, dtype=self.dtype, device=device))
+        if self.nc is None:
+            self.nc = torch.zeros(self.n, dtype=torch.int32, device=device)
+        if self.wh is None:
+            wh64, wh32, K = kernel_tables(self.sincIndex)
+            self.K = K
+            self.wh = torch.from_numpy(wh64 if self.precision == FP64 else wh32).to(device)
+        return self
+
+    def struct(self) -> Particles:
+        s = Particles()
+        s.n, s.precision, s.ng0, s.ngmax, s.Kcour, s.K, s.gamma = (self.n, self.precision, self.ng0, self.ngmax,
+                                                                   self.Kcour, self.K, self.gamma)
+        for f in ("x", "y", "z", "nc", "wh") + self.HYDRO:
+            t = getattr(self, f)
+            setattr(s, f, t.data_ptr() if t is not None else None)
+        s.minDtCourant = self.minDtCourant
+        return s
+
+
+def _groups_struct(groups, d: ParticlesData):
+    if groups is None:
+        return Groups(0, d.n, (d.n + 63) // 64, None, None)
+    return groups
+
+
+def find_neighbors_sfc(ctx: Context, d: ParticlesData, box: Box, octree: OctreeData, groups=None):
+    """sph::findNeighborsSfc: neighbor search with h iteration; d.h and d.nc are updated."""
+    s = d.struct()
+    g = _groups_struct(groups, d)
+    v = octree.view()
+    ctx.check(ctx.lib.sphx_find_neighbors_sph(ctx.h, C.byref(g), C.byref(s), C.byref(box), C.byref(v)))
+
+
+def export_neighbors(ctx: Context, d: ParticlesData):
+    out = torch.zeros(d.n * d.ngmax, dtype=torch.int32, device=d.x.device)
+    ctx.check(ctx.lib.sphx_export_neighbors(ctx.h, _ptr(out)))
+    return out.view(d.n, d.ngmax)
+
+
+def compute_density(ctx: Context, d: ParticlesData, box: Box, groups=None):
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_compute_density(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box)))
+
+
+def compute_eos_hydro_std(ctx: Context, d: ParticlesData, first=0, last=None):
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_compute_eos(ctx.h, first, d.n if last is None else last, C.byref(s)))
+
+
+def compute_iad(ctx: Context, d: ParticlesData, box: Box, groups=None):
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_compute_iad(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box)))
+
+
+def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups=None):
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_compute_momentum_energy(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
+                                                   C.byref(box)))
+    d.minDtCourant = s.minDtCourant
+    return d.minDtCourant
+
+
+# ------------------------------------------------------------------------------------------- the step
+STAGES = ("keys", "sort", "gather", "octree", "link", "centers", "neighbors", "density", "eos", "iad",
+          "momentum_energy")
+
+
+class HydroStep:
+    """One single-rank std-hydro force evaluation from unsorted particles, in the stage
+    order of HydroProp::computeForces (main/src/propagator/std_hydro.hpp:119-168) with
+    the single-rank domain sync (keys, sort, permute, cornerstone tree, link, centres).
+
+    ``compute_forces(inputs)`` takes unsorted device tensors x, y, z (float64) and
+    h, m, vx, vy, vz, u (precision dtype), and leaves the SFC-sorted results in
+    ``self.d`` (a ParticlesData) and the tree in ``self.octree``.
+    """
+
+    def __init__(self, ctx: Context, n: int, box: Box, precision=FP64, bucket_size=64, ng0=100, ngmax=150,
+                 sinc_index=6.0, gamma=5.0 / 3.0, Kcour=0.2):
+        self.ctx, self.n, self.box, self.bucket = ctx, n, box, bucket_size
+        self.d = ParticlesData(n, precision, ng0, ngmax, Kcour, gamma, sinc_index).allocate(ctx.device)
+        self.keys = torch.empty(n, dtype=torch.int64, device=ctx.device)
+        self.order = torch.empty(n, dtype=torch.int32, device=ctx.device)
+        self.octree = None
+        self.events = None
+
+    def compute_forces(self, inp: dict, timed=False):
+        ctx, d, box = self.ctx, self.d, self.box
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(STAGES) + 1)] if timed else None
+        mark = (lambda k: ev[k].record(ctx.stream)) if timed else (lambda k: None)
+        mark(0)
+        self.keys.zero_()
+        compute_sfc_keys(ctx, inp["x"], inp["y"], inp["z"], box, self.keys)
+        mark(1)
+        ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
+        sort_by_key(ctx, self.keys, self.order)
+        mark(2)
+        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u"):
+            gather(ctx, self.order, inp[f], getattr(d, f))
+        mark(3)
+        leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
+        mark(4)
+        o = build_octree(ctx, leaves)
+        o.layout, o.counts = layout, counts
+        mark(5)
+        node_fp_centers(ctx, o, box)
+        self.octree = o
+        mark(6)
+        find_neighbors_sfc(ctx, d, box, o)
+        mark(7)
+        compute_density(ctx, d, box)
+        mark(8)
+        compute_eos_hydro_std(ctx, d)
+        mark(9)
+        compute_iad(ctx, d, box)
+        mark(10)
+        compute_momentum_energy_std(ctx, d, box)
+        mark(11)
+        self.events = ev
+        return d
+
+    def stage_times_ms(self):
+        """Per-stage milliseconds of the last timed compute_forces (synchronises)."""
+        if not self.events:
+            return {}
+        self.events[-1].synchronize()
+        return {s: self.events[k].elapsed_time(self.events[k + 1]) for k, s in enumerate(STAGES)}

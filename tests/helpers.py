@@ -1,0 +1,71 @@
+"""Shared input generators and oracle pipelines for the tests."""
+import numpy as np
+
+import oracle
+
+TABLES = {}
+
+
+def tables():
+    if "t" not in TABLES:
+        TABLES["t"] = oracle.Cpu("oracle").kernel_tables()
+    return TABLES["t"]
+
+
+def uniform_box(n, seed=1, ng=100, periodic=True, lo=0.0, hi=1.0):
+    """i.i.d. uniform positions in a cube, h for ~ng neighbors, v ~ N(0, 0.1^2), u = 1."""
+    rng = np.random.default_rng(seed)
+    L = hi - lo
+    x, y, z = (lo + L * rng.random(n) for _ in range(3))
+    h = np.full(n, 0.5 * L * (ng / (4 / 3 * np.pi * n)) ** (1 / 3))
+    m = np.full(n, 1.0 / n)
+    v = [rng.normal(0, 0.1, n) for _ in range(3)]
+    u = np.ones(n)
+    bc = oracle.BC_PERIODIC if periodic else oracle.BC_OPEN
+    return dict(x=x, y=y, z=z, h=h, m=m, vx=v[0], vy=v[1], vz=v[2], u=u), oracle.Box.cube(lo, hi, bc)
+
+
+def sedov_lattice(side=30):
+    """Sedov blast-wave lattice: side^3 particles at cell centres of [-0.5, 0.5]^3, periodic,
+    h for ~100 neighbors, u = 1e-10 except particles within r < 0.1 sharing total energy 1."""
+    n = side ** 3
+    c = -0.5 + (np.arange(side) + 0.5) / side
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    x, y, z = X.ravel().copy(), Y.ravel().copy(), Z.ravel().copy()
+    m = np.full(n, 1.0 / n)
+    h = np.full(n, 0.5 * (3 * 100 / (4 * np.pi * n)) ** (1 / 3))
+    r = np.sqrt(x * x + y * y + z * z)
+    u = np.full(n, 1e-10)
+    inner = r < 0.1
+    u[inner] = 1.0 / (inner.sum() * m[0])
+    zero = np.zeros(n)
+    return dict(x=x, y=y, z=z, h=h, m=m, vx=zero.copy(), vy=zero.copy(), vz=zero.copy(), u=u), \
+        oracle.Box.cube(-0.5, 0.5, oracle.BC_PERIODIC)
+
+
+def oracle_step(inp, box, ng0=100, ngmax=150, bucket=64, kind="oracle", gamma=5.0 / 3.0, Kcour=0.2):
+    """The reference stage order on the CPU: returns a dict of SFC-sorted results."""
+    cpu = oracle.Cpu(kind)
+    n = len(inp["x"])
+    keys = cpu.sfc_keys(inp["x"], inp["y"], inp["z"], box)
+    keys, order = cpu.sort_by_key(keys, np.arange(n, dtype=np.uint32))
+    s = {k: np.ascontiguousarray(v[order]) for k, v in inp.items()}
+    t = cpu.tree(keys, box, bucket)
+    wh, _, K = cpu.kernel_tables()
+    nb, nc, h = cpu.find_neighbors_sph(s["x"], s["y"], s["z"], s["h"], box, t, ng0, ngmax)
+    rho = cpu.density(nb, nc, ngmax, s["x"], s["y"], s["z"], h, s["m"], wh, K, box)
+    p, c = cpu.eos(s["u"], rho, gamma)
+    cs = cpu.iad(nb, nc, ngmax, s["x"], s["y"], s["z"], h, s["m"], rho, wh, K, box)
+    ax, ay, az, du, dt = cpu.momentum_energy(nb, nc, ngmax, s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"], h,
+                                             s["m"], rho, p, c, cs, wh, K, Kcour, box)
+    return dict(keys=keys, order=order, sorted=s, tree=t, nbrs=nb, nc=nc, h=h, rho=rho, p=p, c=c, c11=cs[0],
+                c12=cs[1], c13=cs[2], c22=cs[3], c23=cs[4], c33=cs[5], ax=ax, ay=ay, az=az, du=du, minDt=dt, K=K,
+                wh=wh)
+
+
+def rel_err(a, b, floor=0.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(b), floor)
+    scale[scale == 0] = 1.0
+    return float(np.max(np.abs(a - b) / scale)) if a.size else 0.0

@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Single-rank SPH hydro step throughput on one MI355X (mixed vs FP64).
+
+A step is one force evaluation from unsorted particles resident in HBM: Hilbert keys,
+radix sort, permutation of the fields, cornerstone octree build and link, node
+centres, neighbor search (with the smoothing-length iteration), density, EOS, IAD
+and momentum-energy, in the stage order of the reference's HydroProp::computeForces.
+
+Workload: subsonic turbulence box, 256^3 = 16,777,216 particles on a jittered
+periodic lattice (sigma = 0.1 spacing), velocities a Gaussian random field with
+P(k) ~ k^(-11/3) for 1 <= |k| <= 3 at rms Mach 0.3, u = 1, h for ~100 neighbors.
+`value` is the mixed-precision step's particles/s; the FP64 step is timed in the same
+run and reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 256]
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "hydro step particles/sec (nbr search+density+IAD+mom-energy), mixed vs FP64"
+
+# published MI355X peaks
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+
+# algorithmic work per unit, counted from the reference pair loops
+# (mul/add/sub/div/sqrt = 1 flop, one table lookup = 7 flops)
+FLOP_PER_PAIR = {"densityKernel": 19.0, "iadKernel": 37.0, "momentumEnergyKernel": 121.0}
+FLOP_PER_TEST = 11.0  # neighbor candidate test, the reference's own convention
+IAD_FLOP_PER_PARTICLE = 60.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup):
+    import torch
+
+    step = hydro.HydroStep(ctx, n, box, precision)
+    dt = step.d.dtype
+    inp = {k: (v if (k in ("x", "y", "z") or dt == torch.float64) else v.to(dt)) for k, v in inp64.items()}
+    for w in range(warmup):
+        step.compute_forces(inp)
+        torch.cuda.synchronize()
+        log(f"precision {precision} warmup {w + 1}/{warmup} done")
+    # one instrumented step for the per-stage breakdown (outside the timed region)
+    step.compute_forces(inp, timed=True)
+    stages = step.stage_times_ms()
+    ctx.set_timing(True)
+    ctx.timing_summary()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step.compute_forces(inp)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kernels = ctx.timing_summary()
+    ctx.set_timing(False)
+    ms = (t1 - t0) * 1e3 / steps
+    log(f"precision {precision}: {ms:.2f} ms/step over {steps} steps")
+    ncs = step.d.nc.to(torch.int64)
+    pairs = int((ncs - 1).clamp(min=0, max=step.d.ngmax).sum().item())
+    return step, ms, stages, kernels, pairs
+
+
+def roofline(kernels, pairs, n, steps, precision, tests):
+    """Achieved flop rate of the dominant kernel from live HIP-event timings."""
+    name, (launches, total) = max(kernels.items(), key=lambda kv: kv[1][1])
+    per_launch_ms = total / max(1, launches)
+    if name in FLOP_PER_PAIR:
+        flop = FLOP_PER_PAIR[name] * pairs + (IAD_FLOP_PER_PARTICLE * n if name == "iadKernel" else 0.0)
+        unit_note = f"{FLOP_PER_PAIR[name]:.0f} flop/pair x {pairs} pairs"
+    elif name == "nsCandidates":
+        flop = FLOP_PER_TEST * tests
+        unit_note = f"{FLOP_PER_TEST:.0f} flop/candidate test x {tests} tests"
+    else:
+        flop = None
+        unit_note = "no algorithmic flop count"
+    peak = PEAK_FP64_TFLOPS if precision == 0 else PEAK_FP32_TFLOPS
+    achieved = (flop / (per_launch_ms * 1e-3) / 1e12) if flop else None
+    return {"kernel": name, "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": (achieved / peak) if achieved else None, "traffic": None, "per_launch_ms": per_launch_ms,
+            "launches": launches, "work": unit_note,
+            "peak_note": "MI355X published vector peak for the kernel's arithmetic (FP32 157.3 for the mixed "
+                         "path, FP64 78.6); MFMA is not used"}
+
+
+def cpu_baseline(side, steps):
+    """The reference CPU path (oracle/_ref, compiled from the reference sources) on a
+    bounded sample of the same workload, on the host cores; the oracle port if the
+    reference build is absent."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle
+    from sphexa_amd.workloads import turbulence
+
+    kind = "reference" if oracle.available("ref") else "port"
+    f, _ = turbulence(side)
+    n = len(f["x"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
+    p = lambda a: np.ascontiguousarray(a).ctypes.data_as(C.c_void_p)
+    times = []
+    if kind == "reference":
+        r = oracle.Cpu("ref")
+        rho, ax, du = np.zeros(n), np.zeros(n), np.zeros(n)
+        nc = np.zeros(n, np.uint32)
+        st = np.zeros(11)
+        for s in range(steps):
+            t0 = time.perf_counter()
+            r._hydro_step(n, p(f["x"]), p(f["y"]), p(f["z"]), p(f["h"]), p(f["m"]), p(f["vx"]), p(f["vy"]),
+                          p(f["vz"]), p(f["u"]), C.byref(box), 64, 100, 150, 6.0, 5.0 / 3.0, 0.2, p(rho), p(ax),
+                          p(du), p(nc), p(st))
+            times.append(time.perf_counter() - t0)
+            log(f"cpu baseline step {s + 1}/{steps}: {times[-1]:.2f} s")
+    else:
+        from tests.helpers import oracle_step
+
+        for s in range(steps):
+            t0 = time.perf_counter()
+            oracle_step(f, box)
+            times.append(time.perf_counter() - t0)
+            log(f"cpu baseline (port) step {s + 1}/{steps}: {times[-1]:.2f} s")
+    t = sorted(times)[len(times) // 2]
+    return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": kind,
+            "sample": f"turbulence {side}^3 = {n} particles (same generator, FP64), median of {steps} full "
+                      f"steps of the reference CPU path (keys, sort, tree, findNeighborsSph, density, EOS, IAD, "
+                      f"momentum-energy), {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side", type=int, default=256, help="lattice side; N = side^3")
+    ap.add_argument("--cpu-side", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
+                    help="time one precision only (for profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              f"(the single-rank path has no multi-GPU decomposition).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import BC_PERIODIC, FP64, MIXED
+    from sphexa_amd.workloads import turbulence
+
+    n = args.side ** 3
+    log(f"generating turbulence {args.side}^3 = {n} particles")
+    ctx = hydro.Context(0)
+    inp64, _ = turbulence(args.side, device=ctx.device)
+    torch.cuda.synchronize()
+    box = hydro.make_box(0.0, 1.0, BC_PERIODIC)
+
+    results = {}
+    for prec in ([MIXED, FP64] if args.only is None else ([MIXED] if args.only == "mixed" else [FP64])):
+        step, ms, stages, kernels, pairs = run_precision(ctx, hydro, inp64, box, n, prec, args.steps, args.warmup)
+        results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs,
+                             rho=step.d.rho.double().clone(), tests=None)
+        del step
+        torch.cuda.empty_cache()
+
+    main_prec = MIXED if MIXED in results else FP64
+    R = results[main_prec]
+    out = {
+        "metric": METRIC,
+        "value": n / (R["ms"] * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_prec == MIXED else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"subsonic turbulence {args.side}^3 jittered periodic lattice ({n} particles), "
+                               f"~100 neighbors, rms Mach 0.3, mixed precision (FP64 positions/offsets, "
+                               f"FP32 weights and Kahan sums)",
+                   "particles": n, "bucket": 64, "ng0": 100, "ngmax": 150, "parallelism": "single GPU"},
+        "pairs_per_s": R["pairs"] / (R["ms"] * 1e-3),
+        "stages_ms": {k: round(v, 3) for k, v in R["stages"].items()},
+        "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in R["kernels"].items()},
+    }
+    if FP64 in results and MIXED in results:
+        F = results[FP64]
+        out["fp64"] = {"value": n / (F["ms"] * 1e-3), "ms_per_step": F["ms"],
+                       "stages_ms": {k: round(v, 3) for k, v in F["stages"].items()},
+                       "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in F["kernels"].items()}}
+        out["mixed_over_fp64"] = F["ms"] / R["ms"]
+        rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
+        out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())
+    out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

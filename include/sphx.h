@@ -130,10 +130,20 @@ const char* sphx_version(void);
 /* context options */
 enum
 {
-    SPHX_OPT_FP64_EXACT = 1 /* 1: FP64 particle loops compiled without FMA contraction, so
-                               every operation rounds like the reference CPU loops (default 0) */
+    SPHX_OPT_FP64_EXACT = 1, /* 1: FP64 particle loops compiled without FMA contraction, so
+                                every operation rounds like the reference CPU loops (default 0) */
+    SPHX_OPT_TIMING     = 2  /* 1: record HIP events on the context stream around every hot
+                                kernel launch (read with sphx_timing_summary) */
 };
 int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value);
+
+/* Per-kernel totals of the launches recorded since the last call (synchronises the
+ * stream, then clears the record): up to `cap` entries of kernel name (static
+ * string), number of launches and summed milliseconds; *numEntries receives the
+ * number of distinct kernels.  Timing is a library service with no counterpart in
+ * the reference (which times whole stages, main/src/util/timer.hpp:34-80). */
+int sphx_timing_summary(sphx_ctx* ctx, int cap, const char** names, int* launches, double* totalMs,
+                        int* numEntries);
 
 /* ---- kernel tables ------------------------------------------------------- */
 /* Host-side table and normalisation: replaces ParticlesData::createTables

@@ -17,6 +17,7 @@ SPHX_OK, SPHX_ERR_HIP, SPHX_ERR_ARG, SPHX_ERR_STACK, SPHX_ERR_NGMAX, SPHX_ERR_ST
 BC_OPEN, BC_PERIODIC, BC_FIXED = 0, 1, 2
 FP64, MIXED = 0, 1
 OPT_FP64_EXACT = 1
+OPT_TIMING = 2
 TABLE_SIZE = 20000
 MAX_LEVEL = 21
 
@@ -83,6 +84,8 @@ SIGNATURES = {
     "sphx_compute_eos": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles)]),
     "sphx_compute_iad": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_momentum_energy": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
+    "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),
+                                      C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -46,6 +46,21 @@ void* scratch(sphx_ctx* ctx, Slot s, size_t bytes)
     return b.p;
 }
 
+hipEvent_t timingEvent(sphx_ctx* ctx)
+{
+    if (ctx->eventsUsed == ctx->eventPool.size())
+    {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ctx->eventPool.push_back(e);
+    }
+    return ctx->eventPool[ctx->eventsUsed++];
+}
+
 bool validBox(const sphx_box* b)
 {
     if (!b) return false;
@@ -325,15 +340,24 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
     for (int shift = 0; shift < keyBits; shift += 8)
     {
         if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
-        radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles);
+        {
+            KTimer kt_(ctx, "radixUpsweep");
+            radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles);
+        }
         SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
         SPHX_TRY(exclusiveScanU32(ctx, hist, hist, uint64_t(RADIX) * numTiles));
         if (vals)
-            radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
+            {
+                KTimer kt_(ctx, "radixScatter");
+                radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
                                                                           numTiles);
+            }
         else
-            radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
+            {
+                KTimer kt_(ctx, "radixScatter");
+                radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
                                                                            hist, numTiles);
+            }
         SPHX_LAUNCH_CHECK(ctx, "radixScatter");
         std::swap(kSrc, kDst);
         std::swap(vSrc, vDst);
@@ -396,7 +420,10 @@ int gatherT(sphx_ctx* ctx, const uint32_t* order, const T* in, T* out, uint64_t 
     if (n == 0) return SPHX_OK;
     if (!order || !in || !out) return argError(ctx, "gather: null pointer");
     if (static_cast<const void*>(in) == static_cast<const void*>(out)) return argError(ctx, "gather: in == out");
-    gatherKernel<T><<<gridFor(n, 256), 256, 0, ctx->stream>>>(order, in, out, n);
+    {
+        KTimer kt_(ctx, "gatherKernel");
+        gatherKernel<T><<<gridFor(n, 256), 256, 0, ctx->stream>>>(order, in, out, n);
+    }
     SPHX_LAUNCH_CHECK(ctx, "gatherKernel");
     return SPHX_OK;
 }
@@ -436,10 +463,48 @@ int sphx_ctx_create(int device, void* stream, sphx_ctx** out)
     return SPHX_OK;
 }
 
+int sphx_timing_summary(sphx_ctx* ctx, int cap, const char** names, int* launches, double* totalMs,
+                        int* numEntries)
+{
+    if (!ctx || !numEntries) return SPHX_ERR_ARG;
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<const char*> nm;
+    std::vector<int>         cnt;
+    std::vector<double>      ms;
+    for (auto& t : ctx->timed)
+    {
+        float e = 0.f;
+        SPHX_HIP(ctx, hipEventElapsedTime(&e, t.a, t.b));
+        size_t k = 0;
+        while (k < nm.size() && std::strcmp(nm[k], t.name) != 0)
+            ++k;
+        if (k == nm.size())
+        {
+            nm.push_back(t.name);
+            cnt.push_back(0);
+            ms.push_back(0.0);
+        }
+        cnt[k] += 1;
+        ms[k] += double(e);
+    }
+    ctx->timed.clear();
+    ctx->eventsUsed = 0;
+    *numEntries     = int(nm.size());
+    for (int k = 0; k < int(nm.size()) && k < cap; ++k)
+    {
+        if (names) names[k] = nm[k];
+        if (launches) launches[k] = cnt[k];
+        if (totalMs) totalMs[k] = ms[k];
+    }
+    return SPHX_OK;
+}
+
 int sphx_ctx_destroy(sphx_ctx* ctx)
 {
     if (!ctx) return SPHX_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->eventPool)
+        (void)hipEventDestroy(e);
     for (auto& b : ctx->buf)
         if (b.p) (void)hipFree(b.p);
     delete ctx;
@@ -467,7 +532,10 @@ int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const
     if (n == 0) return SPHX_OK;
     if (!x || !y || !z || !keys) return argError(ctx, "compute_sfc_keys: null pointer");
     if (!validBox(box)) return argError(ctx, "compute_sfc_keys: invalid box");
-    sfcKeysKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(x, y, z, keys, n, makeDBox(box));
+    {
+        KTimer kt_(ctx, "sfcKeysKernel");
+        sfcKeysKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(x, y, z, keys, n, makeDBox(box));
+    }
     SPHX_LAUNCH_CHECK(ctx, "sfcKeysKernel");
     return SPHX_OK;
 }

@@ -88,6 +88,11 @@ int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value)
         ctx->fp64Exact = value ? 1 : 0;
         return SPHX_OK;
     }
+    if (option == SPHX_OPT_TIMING)
+    {
+        ctx->timing = value ? 1 : 0;
+        return SPHX_OK;
+    }
     return argError(ctx, "ctx_set_option: unknown option");
 }
 
@@ -158,13 +163,19 @@ int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particle
     if (last < first || last > d->n) return argError(ctx, "compute_eos: bad range");
     if (last == first) return SPHX_OK;
     if (d->precision == SPHX_FP64)
-        eosKernel<double><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
+        {
+            KTimer kt_(ctx, "eosKernel");
+            eosKernel<double><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
             first, last, static_cast<const double*>(d->u), static_cast<const double*>(d->rho), d->gamma,
             static_cast<double*>(d->p), static_cast<double*>(d->c));
+        }
     else if (d->precision == SPHX_MIXED)
-        eosKernel<float><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
+        {
+            KTimer kt_(ctx, "eosKernel");
+            eosKernel<float><<<gridFor(last - first, 256), 256, 0, ctx->stream>>>(
             first, last, static_cast<const float*>(d->u), static_cast<const float*>(d->rho), d->gamma,
             static_cast<float*>(d->p), static_cast<float*>(d->c));
+        }
     else
         return argError(ctx, "compute_eos: unknown precision");
     SPHX_LAUNCH_CHECK(ctx, "eosKernel");

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/sphx.h"
 
@@ -101,6 +102,16 @@ struct sphx_ctx
     const uint32_t* nsNc    = nullptr;
     bool            nsValid = false;
     int             fp64Exact = 0; // 1: FP64 loops without FMA contraction (bit-exact vs the CPU path)
+    // per-kernel HIP-event timing (SPHX_OPT_TIMING)
+    struct TimedLaunch
+    {
+        const char* name;
+        hipEvent_t  a, b;
+    };
+    int                      timing = 0;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t>  eventPool;
+    size_t                   eventsUsed = 0;
     // cornerstone results (context-owned)
     int32_t csNumLeaves = 0;
     // computed groups
@@ -146,6 +157,33 @@ T* scratchT(sphx_ctx* ctx, Slot s, size_t count)
     } while (0)
 
 inline unsigned gridFor(uint64_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+// records HIP events around one kernel launch on ctx->stream when timing is enabled
+hipEvent_t timingEvent(sphx_ctx* ctx);
+struct KTimer
+{
+    sphx_ctx*   ctx;
+    const char* name;
+    hipEvent_t  a = nullptr;
+    KTimer(sphx_ctx* c, const char* n)
+        : ctx(c)
+        , name(n)
+    {
+        if (ctx->timing && (a = timingEvent(ctx))) (void)hipEventRecord(a, ctx->stream);
+    }
+    void stop()
+    {
+        if (!a) return;
+        hipEvent_t b = timingEvent(ctx);
+        if (b)
+        {
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->timed.push_back({name, a, b});
+        }
+        a = nullptr;
+    }
+    ~KTimer() { stop(); }
+};
 
 // ---------------------------------------------------------------------------
 // device-wide primitives (sphx_primitives.hip)

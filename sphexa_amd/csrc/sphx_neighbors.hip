@@ -408,12 +408,18 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
     for (int pass = 0; pass < 12 && numLaunch > 0; ++pass)
     {
         uint32_t* outList = redo + size_t(pingpong) * numG;
-        nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, t->childOffsets,
+        {
+            KTimer kt_(ctx, "nsGroupRanges");
+            nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, t->childOffsets,
                                                             t->internalToLeaf, t->layout, t->centers, t->sizes, db,
                                                             ranges, rcnt, flags);
+        }
         SPHX_LAUNCH_CHECK(ctx, "nsGroupRanges");
-        nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, iter, ranges, rcnt,
+        {
+            KTimer kt_(ctx, "nsCandidates");
+            nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, iter, ranges, rcnt,
                                                                 db, ngmax, ng0, lay, nbrs, nc, outList, flags);
+        }
         SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
         NsFlags hf;
         SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));

@@ -238,7 +238,10 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
     if (!sOfs) return SPHX_ERR_NOMEM;
     if (nn > 0)
     {
-        csCountSplits<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs);
+        {
+            KTimer kt_(ctx, "csCountSplits");
+            csCountSplits<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs);
+        }
         SPHX_LAUNCH_CHECK(ctx, "csCountSplits");
         SPHX_TRY(exclusiveScanU32(ctx, sOfs, sOfs, nn));
         SPHX_HIP(ctx, hipMemcpyAsync(&S, sOfs + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -255,13 +258,19 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
     SPHX_HIP(ctx, hipMemsetAsync(child + uint64_t(S) * 7, 0, sizeof(uint64_t), ctx->stream));
     if (S > 0)
     {
-        csEmitChildren<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, child);
+        {
+            KTimer kt_(ctx, "csEmitChildren");
+            csEmitChildren<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, child);
+        }
         SPHX_LAUNCH_CHECK(ctx, "csEmitChildren");
         SPHX_TRY(radixSortPairs(ctx, child, nullptr, L, 64));
     }
     csFinishLeaves<<<1, 64, 0, ctx->stream>>>(child, uint32_t(L));
     SPHX_LAUNCH_CHECK(ctx, "csFinishLeaves");
-    csLayout<<<gridFor(L + 1, 256), 256, 0, ctx->stream>>>(keys, nn, child, uint32_t(L), lay);
+    {
+        KTimer kt_(ctx, "csLayout");
+        csLayout<<<gridFor(L + 1, 256), 256, 0, ctx->stream>>>(keys, nn, child, uint32_t(L), lay);
+    }
     SPHX_LAUNCH_CHECK(ctx, "csLayout");
     csCounts<<<gridFor(L, 256), 256, 0, ctx->stream>>>(lay, uint32_t(L), maxCount, cnts);
     SPHX_LAUNCH_CHECK(ctx, "csCounts");
@@ -297,8 +306,11 @@ int sphx_build_octree(sphx_ctx* ctx, const uint64_t* leaves, sphx_octree* t)
     t->numInternalNodes = ni;
     t->numNodes         = nn;
 
-    octUnsortedLayout<<<gridFor(L, 256), 256, 0, ctx->stream>>>(leaves, ni, L, t->prefixes,
+    {
+        KTimer kt_(ctx, "octUnsortedLayout");
+        octUnsortedLayout<<<gridFor(L, 256), 256, 0, ctx->stream>>>(leaves, ni, L, t->prefixes,
                                                                 reinterpret_cast<uint32_t*>(t->internalToLeaf));
+    }
     SPHX_LAUNCH_CHECK(ctx, "octUnsortedLayout");
     SPHX_TRY(radixSortPairs(ctx, t->prefixes, reinterpret_cast<uint32_t*>(t->internalToLeaf), uint64_t(nn), 64));
     octInvert<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(t->internalToLeaf, t->leafToInternal, nn, ni);
@@ -309,8 +321,11 @@ int sphx_build_octree(sphx_ctx* ctx, const uint64_t* leaves, sphx_octree* t)
     SPHX_LAUNCH_CHECK(ctx, "octZero");
     if (ni > 0)
     {
-        octLink<<<gridFor(ni, 256), 256, 0, ctx->stream>>>(t->prefixes, ni, t->leafToInternal, t->levelRange,
+        {
+            KTimer kt_(ctx, "octLink");
+            octLink<<<gridFor(ni, 256), 256, 0, ctx->stream>>>(t->prefixes, ni, t->leafToInternal, t->levelRange,
                                                            t->childOffsets, t->parents);
+        }
         SPHX_LAUNCH_CHECK(ctx, "octLink");
     }
     else
@@ -329,7 +344,10 @@ int sphx_node_centers(sphx_ctx* ctx, const uint64_t* prefixes, int32_t numNodes,
     if (numNodes <= 0) return SPHX_OK;
     if (!prefixes || !centers || !sizes) return argError(ctx, "node_centers: null pointer");
     if (!validBox(box)) return argError(ctx, "node_centers: invalid box");
-    octCenters<<<gridFor(numNodes, 256), 256, 0, ctx->stream>>>(prefixes, numNodes, makeDBox(box), centers, sizes);
+    {
+        KTimer kt_(ctx, "octCenters");
+        octCenters<<<gridFor(numNodes, 256), 256, 0, ctx->stream>>>(prefixes, numNodes, makeDBox(box), centers, sizes);
+    }
     SPHX_LAUNCH_CHECK(ctx, "octCenters");
     return SPHX_OK;
 }

@@ -83,6 +83,20 @@ class Context:
     def set_fp64_exact(self, on: bool):
         self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_FP64_EXACT, int(bool(on))))
 
+    def set_timing(self, on: bool):
+        """Record HIP events around every hot kernel launch (read with timing_summary)."""
+        self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_TIMING, int(bool(on))))
+
+    def timing_summary(self):
+        """{kernel: (launches, total ms)} since the last call; synchronises the stream."""
+        cap = 64
+        names = (C.c_char_p * cap)()
+        launches = (C.c_int * cap)()
+        ms = (C.c_double * cap)()
+        num = C.c_int()
+        self.check(self.lib.sphx_timing_summary(self.h, cap, names, launches, ms, C.byref(num)))
+        return {names[k].decode(): (launches[k], ms[k]) for k in range(min(num.value, cap))}
+
     def synchronize(self):
         self.check(self.lib.sphx_ctx_synchronize(self.h))
 

@@ -70,7 +70,8 @@ def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup):
     log(f"precision {precision}: {ms:.2f} ms/step over {steps} steps")
     ncs = step.d.nc.to(torch.int64)
     pairs = int((ncs - 1).clamp(min=0, max=step.d.ngmax).sum().item())
-    return step, ms, stages, kernels, pairs
+    tests = ctx.candidate_tests()
+    return step, ms, stages, kernels, pairs, tests
 
 
 def roofline(kernels, pairs, n, steps, precision, tests):
@@ -80,7 +81,7 @@ def roofline(kernels, pairs, n, steps, precision, tests):
     if name in FLOP_PER_PAIR:
         flop = FLOP_PER_PAIR[name] * pairs + (IAD_FLOP_PER_PARTICLE * n if name == "iadKernel" else 0.0)
         unit_note = f"{FLOP_PER_PAIR[name]:.0f} flop/pair x {pairs} pairs"
-    elif name == "nsCandidates":
+    elif name == "nsCandidates" and tests:
         flop = FLOP_PER_TEST * tests
         unit_note = f"{FLOP_PER_TEST:.0f} flop/candidate test x {tests} tests"
     else:
@@ -172,9 +173,10 @@ def main():
 
     results = {}
     for prec in ([MIXED, FP64] if args.only is None else ([MIXED] if args.only == "mixed" else [FP64])):
-        step, ms, stages, kernels, pairs = run_precision(ctx, hydro, inp64, box, n, prec, args.steps, args.warmup)
+        step, ms, stages, kernels, pairs, tests = run_precision(ctx, hydro, inp64, box, n, prec, args.steps,
+                                                                args.warmup)
         results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs,
-                             rho=step.d.rho.double().clone(), tests=None)
+                             rho=step.d.rho.double().clone(), tests=tests)
         del step
         torch.cuda.empty_cache()
 

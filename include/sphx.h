@@ -132,10 +132,21 @@ enum
 {
     SPHX_OPT_FP64_EXACT = 1, /* 1: FP64 particle loops compiled without FMA contraction, so
                                 every operation rounds like the reference CPU loops (default 0) */
-    SPHX_OPT_TIMING     = 2  /* 1: record HIP events on the context stream around every hot
+    SPHX_OPT_TIMING     = 2, /* 1: record HIP events on the context stream around every hot
                                 kernel launch (read with sphx_timing_summary) */
+    SPHX_OPT_LOOP_SHAPE_FP64  = 3, /* particle-loop kernel shape for FP64 fields:          */
+    SPHX_OPT_LOOP_SHAPE_MIXED = 4  /* 0 = neighbor records staged in LDS per 16-target     */
+                                   /* sub-group (default), 1 = one lane per target with    */
+                                   /* neighbor records gathered from global memory         */
 };
 int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value);
+
+/* statistics of the last call that produced them */
+enum
+{
+    SPHX_STAT_CANDIDATE_TESTS = 1 /* (target, candidate) pairs tested by the last neighbor search */
+};
+int sphx_get_stat(sphx_ctx* ctx, int stat, double* value);
 
 /* Per-kernel totals of the launches recorded since the last call (synchronises the
  * stream, then clears the record): up to `cap` entries of kernel name (static

@@ -94,6 +94,7 @@ class Cpu:
             f("eos", None, [_U, _U, _V, _V, _D, _V, _V])
             f("iad", None, [_U, _U, _V, _V, _U] + [_V] * 7 + [_D, _V] + [_V] * 6)
             f("momentum_energy", _D, [_U, _U, _V, _V, _U] + [_V] * 18 + [_D, C.c_float, _V] + [_V] * 4)
+            f("compute_groups", _I, [_U, _U, _V, _V, _V, _V, _V, _I, _V, _V, C.c_float, _V, _I])
             f("iad_one", None, [_U, _V, _U] + [_V] * 7 + [_D, _V, _V])
             f("momentum_energy_one", None, [_U, _V, _U] + [_V] * 18 + [_D, _V, _V])
             f("rebalance_decision", _I, [_V, _V, _I, _U, _V])
@@ -191,6 +192,18 @@ class Cpu:
         c, s = self.node_centers(o["prefixes"], box)
         o.update(leaves=leaves, counts=counts, layout=layout, centers=c, sizes=s)
         return o
+
+    def compute_groups(self, x, y, z, h, leaves, layout, box, tol=2.0, first=0, last=None):
+        """Target-group boundaries (numGroups + 1 entries) of computeGroupSplits with groupSize 64."""
+        last = len(x) if last is None else last
+        cap = 64 * ((last - first + 63) // 64) + 2
+        b = np.zeros(cap, np.uint32)
+        hh = None if h is None else np.ascontiguousarray(h, np.float64)
+        ng = self._compute_groups(first, last, _p(x), _p(y), _p(z), _p(hh), _p(np.ascontiguousarray(leaves, np.uint64)),
+                                  len(leaves) - 1, _p(np.ascontiguousarray(layout, np.uint32)), C.byref(box), tol,
+                                  _p(b), cap)
+        assert ng >= 0
+        return b[:ng + 1]
 
     # ---------------------------------------------------------------- neighbors
     def find_neighbors(self, x, y, z, h, box, t, ngmax, first=0, last=None, lists=True):

@@ -961,4 +961,49 @@ double orc_xmass_one(uint32_t i, const uint32_t* nb, unsigned cnt, const double*
     return mi / (rho0i * K * h3Inv);
 }
 
+// computeGroupSplits semantics (domain/include/cstone/traversal/groups_gpu.cuh:37-77,
+// 91-114, 148-216; groups_gpu.cu:108-137) with groupSize 64: writes the group
+// boundaries (numGroups + 1 entries, cap checked) and returns numGroups (-1 if cap too small)
+int orc_compute_groups(uint32_t first, uint32_t last, const double* x, const double* y, const double* z,
+                       const double* h, const uint64_t* leaves, int numLeaves, const uint32_t* layout,
+                       const sphx_box* b, float tolFactor, uint32_t* bounds, int cap)
+{
+    Box box = makeBox(b);
+    double minExt = std::min(std::min(box.len[0], box.len[1]), box.len[2]);
+    uint32_t numFixed = (last - first + 63) / 64;
+    std::vector<uint32_t> out;
+    for (uint32_t g = 0; g < numFixed; ++g)
+    {
+        double pos[64][4];
+        double nodeVolume = 1.0;
+        for (int lane = 0; lane < 64; ++lane)
+        {
+            uint32_t bi = std::min(first + 64 * g + lane, last - 1);
+            int leaf = int(std::upper_bound(layout, layout + numLeaves, bi) - layout) - 1;
+            unsigned level = treeLevel(leaves[leaf + 1] - leaves[leaf]);
+            double s = std::ldexp(0.5, -int(level)); // centerAndSize half-size of the leaf in the unit box
+            nodeVolume = std::min(nodeVolume, 8 * s * s * s);
+            pos[lane][0] = x[bi] * box.ilen[0];
+            pos[lane][1] = y[bi] * box.ilen[1];
+            pos[lane][2] = z[bi] * box.ilen[2];
+            pos[lane][3] = h ? 2.0 * h[bi] / minExt : 1.0;
+        }
+        double distCrit = std::cbrt(nodeVolume) * double(tolFactor);
+        out.push_back(first + 64 * g);
+        for (int lane = 0; lane < 63; ++lane)
+        {
+            double dx = pos[lane + 1][0] - pos[lane][0], dy = pos[lane + 1][1] - pos[lane][1],
+                   dz = pos[lane + 1][2] - pos[lane][2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > std::min(distCrit * distCrit, pos[lane][3] * pos[lane][3])) out.push_back(first + 64 * g + lane + 1);
+        }
+    }
+    out.push_back(last);
+    int numGroups = int(out.size()) - 1;
+    if (numFixed == 0) numGroups = 0;
+    if (cap < numGroups + 1) return -1;
+    std::copy(out.begin(), out.end(), bounds);
+    return numGroups;
+}
+
 } // extern "C"

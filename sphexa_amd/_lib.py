@@ -18,6 +18,9 @@ BC_OPEN, BC_PERIODIC, BC_FIXED = 0, 1, 2
 FP64, MIXED = 0, 1
 OPT_FP64_EXACT = 1
 OPT_TIMING = 2
+OPT_LOOP_SHAPE_FP64 = 3
+OPT_LOOP_SHAPE_MIXED = 4
+STAT_CANDIDATE_TESTS = 1
 TABLE_SIZE = 20000
 MAX_LEVEL = 21
 
@@ -84,6 +87,7 @@ SIGNATURES = {
     "sphx_compute_eos": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles)]),
     "sphx_compute_iad": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_momentum_energy": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
+    "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),
                                       C.POINTER(C.c_int)]),
 }

@@ -15,11 +15,10 @@ namespace sphx
 {
 namespace exact
 {
-int launchDensity(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const DBox&);
-int launchPrep(sphx_ctx*, uint32_t, uint32_t, sphx_particles*, void*, void*, void*);
-int launchIAD(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const void*, const DBox&);
-int launchMomentumEnergy(sphx_ctx*, const GroupsDev&, const uint32_t*, sphx_particles*, const void*, const void*,
-                         const void*, const DBox&, unsigned long long*);
+int launchDensity(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&);
+int launchIAD(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&);
+int launchMomentumEnergy(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&,
+                         unsigned long long*);
 } // namespace exact
 
 int groupsFromArgs(sphx_ctx* ctx, const sphx_groups* groups, uint32_t first, uint32_t last, GroupsDev& G);
@@ -51,7 +50,7 @@ static bool useExact(const sphx_ctx* ctx, const sphx_particles* d)
 // the SPH loops run on the lists of the last sphx_find_neighbors_sph call and
 // require the same groups
 static int listsFor(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d, GroupsDev& G,
-                    const uint32_t*& lists)
+                    SphLists& lists)
 {
     if (!ctx->nsValid) return fail(ctx, SPHX_ERR_STATE, "no neighbor lists: call sphx_find_neighbors_sph first");
     SPHX_TRY(groupsFromArgs(ctx, groups, 0, uint32_t(d->n), G));
@@ -59,7 +58,7 @@ static int listsFor(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particl
     if (G.first != N.first || G.last != N.last || G.numGroups != N.numGroups || G.start != N.start ||
         G.end != N.end || d->ngmax != ctx->nsNgmax || d->nc != ctx->nsNc)
         return fail(ctx, SPHX_ERR_STATE, "groups, ngmax or nc differ from the neighbor search");
-    lists = static_cast<const uint32_t*>(ctx->buf[S_NS_LISTS].p);
+    lists = ctx->nsLists;
     return SPHX_OK;
 }
 
@@ -71,7 +70,6 @@ static int checkParticles(sphx_ctx* ctx, const sphx_particles* d, const char* wh
     return SPHX_OK;
 }
 
-static size_t elemSize(const sphx_particles* d) { return d->precision == SPHX_FP64 ? 8 : 4; }
 
 } // namespace sphx
 
@@ -93,7 +91,24 @@ int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value)
         ctx->timing = value ? 1 : 0;
         return SPHX_OK;
     }
+    if (option == SPHX_OPT_LOOP_SHAPE_FP64 || option == SPHX_OPT_LOOP_SHAPE_MIXED)
+    {
+        if (value != 0 && value != 1) return argError(ctx, "ctx_set_option: loop shape must be 0 or 1");
+        ctx->loopShape[option == SPHX_OPT_LOOP_SHAPE_FP64 ? 0 : 1] = value;
+        return SPHX_OK;
+    }
     return argError(ctx, "ctx_set_option: unknown option");
+}
+
+int sphx_get_stat(sphx_ctx* ctx, int stat, double* value)
+{
+    if (!ctx || !value) return SPHX_ERR_ARG;
+    if (stat == SPHX_STAT_CANDIDATE_TESTS)
+    {
+        *value = ctx->nsTests;
+        return SPHX_OK;
+    }
+    return argError(ctx, "get_stat: unknown statistic");
 }
 
 int sphx_kernel_tables(double sincIndex, double* wh64, float* wh32, double* K)
@@ -149,7 +164,7 @@ int sphx_compute_density(sphx_ctx* ctx, const sphx_groups* groups, sphx_particle
     if (!d->rho) return argError(ctx, "compute_density: null rho");
     if (!validBox(box)) return argError(ctx, "compute_density: invalid box");
     GroupsDev       G;
-    const uint32_t* lists;
+    SphLists        lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     if (G.numGroups == 0) return SPHX_OK;
     DBox db = makeDBox(box);
@@ -190,15 +205,11 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
         return argError(ctx, "compute_iad: null field");
     if (!validBox(box)) return argError(ctx, "compute_iad: invalid box");
     GroupsDev       G;
-    const uint32_t* lists;
+    SphLists        lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     if (G.numGroups == 0) return SPHX_OK;
-    void* vol = scratch(ctx, S_HY_VOL, elemSize(d) * d->n + 256);
-    if (!vol) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
-    // m_j / rho_j is needed for every neighbor j, which may lie outside [first, last)
-    SPHX_TRY(fast::launchPrep(ctx, 0, uint32_t(d->n), d, nullptr, vol, nullptr));
-    return useExact(ctx, d) ? exact::launchIAD(ctx, G, lists, d, vol, db) : fast::launchIAD(ctx, G, lists, d, vol, db);
+    return useExact(ctx, d) ? exact::launchIAD(ctx, G, lists, d, db) : fast::launchIAD(ctx, G, lists, d, db);
 }
 
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
@@ -210,22 +221,17 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
         return argError(ctx, "compute_momentum_energy: null field");
     if (!validBox(box)) return argError(ctx, "compute_momentum_energy: invalid box");
     GroupsDev       G;
-    const uint32_t* lists;
+    SphLists        lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     d->minDtCourant = INFINITY;
     if (G.numGroups == 0) return SPHX_OK;
-    size_t es   = elemSize(d) * d->n + 256;
-    void*  vol  = scratch(ctx, S_HY_VOL, es);
-    void*  hinv = scratch(ctx, S_HY_HINV, es);
-    void*  prho = scratch(ctx, S_HY_PRHO, es);
-    auto*  dt   = scratchT<unsigned long long>(ctx, S_SMALL, 4);
-    if (!vol || !hinv || !prho || !dt) return SPHX_ERR_NOMEM;
+    auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
+    if (!dt) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
-    SPHX_TRY(fast::launchPrep(ctx, 0, uint32_t(d->n), d, hinv, vol, prho));
     initMinDt<<<1, 64, 0, ctx->stream>>>(dt);
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
-    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, hinv, vol, prho, db, dt)
-                              : fast::launchMomentumEnergy(ctx, G, lists, d, hinv, vol, prho, db, dt));
+    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt)
+                              : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt));
     unsigned long long bits = 0;
     SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
     SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));

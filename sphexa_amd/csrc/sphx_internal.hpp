@@ -32,7 +32,9 @@ enum Slot : int
     S_CS_LAYOUT,     // cornerstone: leaf layout
     S_NS_RANGES,     // neighbor search: candidate particle ranges per group
     S_NS_RCOUNT,     // neighbor search: range count per group
-    S_NS_LISTS,      // neighbor lists [group][k][lane]
+    S_NS_LISTS,      // neighbor slot lists [sub-group][k][16]
+    S_NS_UNION,      // neighbor unions per sub-group
+    S_NS_USIZE,      // union sizes
     S_NS_STATE,      // per-particle h-iteration counters / active flags
     S_NS_REDO,       // groups that need another h iteration
     S_GROUPS,        // computed group boundaries
@@ -40,9 +42,7 @@ enum Slot : int
     S_SMALL,         // small device scalars (flags, counters, reductions)
     S_TABLE64,       // sinc table, FP64
     S_TABLE32,       // sinc table, FP32
-    S_HY_VOL,        // m / rho per particle
-    S_HY_HINV,       // 1 / h per particle
-    S_HY_PRHO,       // p / rho per particle
+    S_HY_REC,        // packed neighbor-side records of the SPH loops
     S_NUM_SLOTS
 };
 
@@ -87,6 +87,22 @@ __host__ __device__ inline size_t listBase(const ListLayout& lay, const GroupsDe
     return lay.cpu ? size_t(s + lane - G.first) * ngmax : size_t(g) * ngmax * 64 + lane;
 }
 
+// SPH neighbor lists: per 16-target sub-group sg, the sorted union of its targets'
+// stored neighbors and per target the u16 positions of its neighbors in the union
+struct SphLists
+{
+    uint16_t* slots  = nullptr; // [sg][k][16]
+    uint32_t* unions = nullptr; // [sg][capU]
+    uint32_t* usize  = nullptr; // [sg]
+    uint32_t  capU   = 0;       // 16 * ngmax
+};
+
+// entries of a particle's list that the loops read: min(nc - 1, ngmax), 0 when nc == 0
+__host__ __device__ inline uint32_t listCount(uint32_t ncSelf, uint32_t ngmax)
+{
+    return ncSelf ? (ncSelf - 1u < ngmax ? ncSelf - 1u : ngmax) : 0u;
+}
+
 } // namespace sphx
 
 struct sphx_ctx
@@ -98,10 +114,13 @@ struct sphx_ctx
     sphx::DevBuf    buf[sphx::S_NUM_SLOTS];
     // neighbor-list bookkeeping: the SPH loops require lists of the same particle set
     sphx::GroupsDev nsGroups;
+    sphx::SphLists  nsLists;
     uint32_t        nsNgmax = 0;
     const uint32_t* nsNc    = nullptr;
     bool            nsValid = false;
     int             fp64Exact = 0; // 1: FP64 loops without FMA contraction (bit-exact vs the CPU path)
+    int             loopShape[2] = {0, 0}; // per precision (FP64, MIXED): 0 LDS-staged, 1 sequential
+    double          nsTests      = 0;      // candidate tests of the last neighbor search
     // per-kernel HIP-event timing (SPHX_OPT_TIMING)
     struct TimedLaunch
     {

@@ -28,10 +28,11 @@ constexpr int NS_LCAP = 1024; // candidate leaves per group
 
 struct NsFlags
 {
-    uint32_t redoCount;   // groups appended to the redo list
-    uint32_t overflow;    // traversal capacity exceeded
-    uint32_t badGroup;    // a group with more than 64 particles
-    uint32_t pad;
+    uint32_t           redoCount; // groups appended to the redo list
+    uint32_t           overflow;  // traversal capacity exceeded
+    uint32_t           badGroup;  // a group with more than 64 particles
+    uint32_t           pad;
+    unsigned long long tests;     // candidate pair tests (targets x candidates) of this pass
 };
 
 __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
@@ -52,7 +53,6 @@ template<class Th>
 __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                     const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, const Th* __restrict__ h,
-                                                    const uint8_t* __restrict__ active,
                                                     const int32_t* __restrict__ childOffsets,
                                                     const int32_t* __restrict__ internalToLeaf,
                                                     const uint32_t* __restrict__ layout,
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
         e = s + 64;
     }
     uint32_t i   = s + lane;
-    bool     act = (i < e) && (!active || active[i]);
+    bool     act = i < e;
 
     double lo[3], hi[3];
     {
@@ -216,6 +216,15 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
         rcount += int(popc64(mS));
     }
     if (lane == 0) rangeCount[g] = uint32_t(rcount);
+    // candidate tests the candidate kernel will make for this group
+    unsigned long long cand = 0;
+    for (int r = int(lane); r < rcount; r += 64)
+    {
+        uint2 rg = ranges[size_t(g) * NS_LCAP + r];
+        cand += rg.y - rg.x;
+    }
+    cand = waveSum(cand);
+    if (lane == 0) atomicAdd(&flags->tests, cand * (unsigned long long)(e - s));
 }
 
 // updateH (sph/kernels.hpp:27-33)
@@ -225,15 +234,22 @@ __device__ inline double updateH(unsigned ng0, unsigned nc, double h)
 }
 
 // Candidate test.  Sph = true: findNeighborsSph semantics (nc counts self, h updates,
-// redo flags); Sph = false: findNeighbors semantics (count without self).
+// redo flags) and the SPH list format below; Sph = false: findNeighbors semantics
+// (count without self, u32 lists in the reference CPU layout).
+//
+// SPH list format, per sub-group of 16 consecutive targets (quarter q of the wave,
+// sg = 4 g + q): the sorted union of the stored neighbors of its 16 targets
+// (unions[sg][0..usize)) and per target the ascending u16 positions of its
+// neighbors in that union (slots[sg][k][t]).  Candidates arrive in ascending j, so
+// a union entry is appended the first time any target of the quarter hits it, and
+// slot order equals j order.
 template<class Th, bool Sph>
 __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                    const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, Th* __restrict__ h,
-                                                   uint8_t* __restrict__ active, uint8_t* __restrict__ iter,
-                                                   const uint2* __restrict__ ranges,
+                                                   uint8_t* __restrict__ iter, const uint2* __restrict__ ranges,
                                                    const uint32_t* __restrict__ rangeCount, DBox box, uint32_t ngmax,
-                                                   uint32_t ng0, ListLayout lay, uint32_t* __restrict__ nbrs,
+                                                   uint32_t ng0, uint32_t* __restrict__ nbrs, SphLists out,
                                                    uint32_t* __restrict__ nc, uint32_t* __restrict__ redoList,
                                                    NsFlags* __restrict__ flags)
 {
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     groupRange(G, g, s, e);
     if (e - s > 64) e = s + 64;
     const uint32_t i   = s + lane;
-    const bool     act = (i < e) && (!active || active[i]);
+    const bool     act = i < e;
 
     double xi = 0, yi = 0, zi = 0, hi = 0;
     if (act) xi = x[i], yi = y[i], zi = z[i], hi = double(h[i]);
@@ -258,13 +274,20 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
         for (int d = 0; d < 3; ++d)
             inside = inside && (p[d] - 2.0 * hi >= box.lo[d]) && (p[d] + 2.0 * hi <= box.hi[d]);
     }
-    const bool usePbc = box.anyPbc && !inside;
-    const double px = box.pbc[0] ? box.len[0] : 0.0;
-    const double py = box.pbc[1] ? box.len[1] : 0.0;
-    const double pz = box.pbc[2] ? box.len[2] : 0.0;
+    const bool   usePbc = box.anyPbc && !inside;
+    const double px     = box.pbc[0] ? box.len[0] : 0.0;
+    const double py     = box.pbc[1] ? box.len[1] : 0.0;
+    const double pz     = box.pbc[2] ? box.len[2] : 0.0;
 
-    size_t   lbase = listBase(lay, G, g, s, lane, ngmax);
-    uint32_t count = 0;
+    // non-SPH: reference CPU layout nbrs[(i - first) * ngmax + k]
+    const size_t lbase = size_t(s + lane - G.first) * ngmax;
+    // SPH: this lane's sub-group
+    const unsigned q      = lane >> 4, t = lane & 15u;
+    const uint32_t sg     = 4u * g + q;
+    uint16_t*      mySlot = Sph ? out.slots + size_t(sg) * ngmax * 16 + t : nullptr;
+    uint32_t*      myUni  = Sph ? out.unions + size_t(sg) * out.capU : nullptr;
+    uint32_t       usz    = 0;
+    uint32_t       count  = 0;
 
     const uint32_t R = rangeCount[g];
     for (uint32_t r = 0; r < R; ++r)
@@ -280,27 +303,33 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                 sz[lane] = z[base + lane];
             }
             __syncthreads();
-            if (act)
+            for (uint32_t k = 0; k < m; ++k)
             {
-                for (uint32_t k = 0; k < m; ++k)
+                double dx = sx[k] - xi;
+                double dy = sy[k] - yi;
+                double dz = sz[k] - zi;
+                if (usePbc)
                 {
-                    double dx = sx[k] - xi;
-                    double dy = sy[k] - yi;
-                    double dz = sz[k] - zi;
-                    if (usePbc)
-                    {
-                        dx -= px * rint(dx * box.ilen[0]);
-                        dy -= py * rint(dy * box.ilen[1]);
-                        dz -= pz * rint(dz * box.ilen[2]);
-                    }
-                    double   d2 = dx * dx + dy * dy + dz * dz;
-                    uint32_t j  = base + k;
-                    if (d2 < radiusSq && j != i)
-                    {
-                        if (nbrs && count < ngmax) nbrs[lbase + size_t(count) * lay.strideK] = j;
-                        ++count;
-                    }
+                    dx -= px * rint(dx * box.ilen[0]);
+                    dy -= py * rint(dy * box.ilen[1]);
+                    dz -= pz * rint(dz * box.ilen[2]);
                 }
+                double   d2  = dx * dx + dy * dy + dz * dz;
+                uint32_t j   = base + k;
+                bool     hit = act && d2 < radiusSq && j != i;
+                if constexpr (Sph)
+                {
+                    bool     store = hit && count < ngmax;
+                    uint32_t mq    = uint32_t(ballot(store) >> (16u * q)) & 0xFFFFu;
+                    if (store) mySlot[size_t(count) * 16] = uint16_t(usz);
+                    if (mq && t == 0) myUni[usz] = j;
+                    usz += (mq != 0u);
+                }
+                else
+                {
+                    if (hit && nbrs && count < ngmax) nbrs[lbase + count] = j;
+                }
+                count += hit ? 1u : 0u;
             }
             __syncthreads();
         }
@@ -308,20 +337,20 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
 
     if constexpr (Sph)
     {
+        if (t == 0) out.usize[sg] = usz;
         bool redo = false;
         if (act)
         {
             uint32_t ncSph = count + 1;
             uint32_t ngmin = ng0 / 4;
-            bool     out   = (ngmin > ncSph) || (ncSph - 1 > ngmax);
+            bool     outR  = (ngmin > ncSph) || (ncSph - 1 > ngmax);
             nc[i] = ncSph; // dataset counts are indexed by particle (sph/find_neighbors.hpp:51)
-            if (out && iter[i] < 10)
+            if (outR && iter[i] < 10)
             {
                 h[i] = Th(updateH(ng0, ncSph, hi));
                 iter[i] += 1;
                 redo = true;
             }
-            active[i] = redo ? 1 : 0;
         }
         uint64_t anyRedo = ballot(redo);
         if (anyRedo && lane == 0)
@@ -336,12 +365,11 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     }
 }
 
-__global__ __launch_bounds__(256) void nsInitState(uint8_t* active, uint8_t* iter, uint32_t first, uint32_t last)
+__global__ __launch_bounds__(256) void nsInitState(uint8_t* iter, uint32_t first, uint32_t last)
 {
     uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= last) return;
-    active[i] = 1;
-    iter[i]   = 0;
+    iter[i] = 0;
 }
 
 __global__ void nsResetFlags(NsFlags* f, int clearErrors)
@@ -349,6 +377,7 @@ __global__ void nsResetFlags(NsFlags* f, int clearErrors)
     if (threadIdx.x == 0)
     {
         f->redoCount = 0;
+        f->tests     = 0;
         if (clearErrors)
         {
             f->overflow = 0;
@@ -357,8 +386,8 @@ __global__ void nsResetFlags(NsFlags* f, int clearErrors)
     }
 }
 
-// export [group][k][lane] lists into the reference CPU layout [i - first][k]
-__global__ __launch_bounds__(256) void nsExport(GroupsDev G, const uint32_t* __restrict__ lists, uint32_t ngmax,
+// export the SPH lists into the reference CPU layout [i - first][k]
+__global__ __launch_bounds__(256) void nsExport(GroupsDev G, SphLists L, uint32_t ngmax,
                                                 const uint32_t* __restrict__ nc, uint32_t* __restrict__ out)
 {
     uint32_t g    = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -368,62 +397,136 @@ __global__ __launch_bounds__(256) void nsExport(GroupsDev G, const uint32_t* __r
     groupRange(G, g, s, e);
     uint32_t i = s + lane;
     if (i >= e || lane >= 64) return;
-    uint32_t cnt = nc[i] - 1u;
-    cnt          = min(cnt, ngmax);
-    size_t base  = size_t(g) * ngmax * 64 + lane;
+    uint32_t        cnt  = listCount(nc[i], ngmax);
+    uint32_t        sg   = 4u * g + (lane >> 4);
+    const uint16_t* sl   = L.slots + size_t(sg) * ngmax * 16 + (lane & 15u);
+    const uint32_t* uni  = L.unions + size_t(sg) * L.capU;
     for (uint32_t k = 0; k < cnt; ++k)
-        out[size_t(i - G.first) * ngmax + k] = lists[base + size_t(k) * 64];
+        out[size_t(i - G.first) * ngmax + k] = uni[sl[size_t(k) * 16]];
+}
+
+
+// ---------------------------------------------------------------------------
+// target groups: computeGroupSplits (domain/include/cstone/traversal/groups_gpu.cu:108-137,
+// groups_gpu.cuh:37-77,148-216).  Fixed 64-particle groups are split after particle p
+// when |X(p+1) - X(p)|^2 > min(distCrit^2, (2 h_p / minExtent)^2) in box-length units,
+// distCrit = tolFactor * cbrt(volume of the smallest leaf holding a particle of the group).
+template<class Th>
+__global__ __launch_bounds__(64) void grpSplitMasks(uint32_t first, uint32_t last, const double* __restrict__ x,
+                                                    const double* __restrict__ y, const double* __restrict__ z,
+                                                    const Th* __restrict__ h, const uint64_t* __restrict__ leaves,
+                                                    int numLeaves, const uint32_t* __restrict__ layout, DBox box,
+                                                    float tolFactor, uint64_t* __restrict__ masks,
+                                                    uint32_t* __restrict__ numSegments)
+{
+#pragma clang fp contract(off)
+    const uint32_t g    = blockIdx.x;
+    const unsigned lane = threadIdx.x;
+    uint32_t       bi   = min(first + 64u * g + lane, last - 1);
+    // leaf containing bi: last leaf with layout <= bi
+    int lo = 0, cnt = numLeaves;
+    while (cnt > 0)
+    {
+        int step = cnt / 2, mid = lo + step;
+        if (layout[mid] <= bi)
+        {
+            lo = mid + 1;
+            cnt -= step + 1;
+        }
+        else { cnt = step; }
+    }
+    int      leaf  = lo - 1;
+    unsigned level = treeLevel(leaves[leaf + 1] - leaves[leaf]);
+    // unit-box leaf volume 2^(-3 level); its cube root 2^(-level) is exact
+    int      maxLevel = int(waveMax(int(level)));
+    double   distCrit = ldexp(1.0, -maxLevel) * double(tolFactor);
+    double   minExt   = fmin(fmin(box.len[0], box.len[1]), box.len[2]);
+    double   px = x[bi] * box.ilen[0], py = y[bi] * box.ilen[1], pz = z[bi] * box.ilen[2];
+    double   pw = h ? 2.0 * double(h[bi]) / minExt : 1.0;
+    double   qx = __shfl_down(px, 1, 64), qy = __shfl_down(py, 1, 64), qz = __shfl_down(pz, 1, 64);
+    if (lane == 63) qx = px, qy = py, qz = pz;
+    double   dx = qx - px, dy = qy - py, dz = qz - pz;
+    double   d2 = dx * dx + dy * dy + dz * dz;
+    double   crit2 = distCrit * distCrit;
+    bool     split = d2 > fmin(crit2, pw * pw);
+    uint64_t m     = ballot(split);
+    if (lane == 0)
+    {
+        masks[g]       = m;
+        numSegments[g] = 1u + popc64(m);
+    }
+}
+
+// group boundaries from the split masks (makeSplits, groups_gpu.cuh:91-114)
+__global__ __launch_bounds__(256) void grpBoundaries(uint32_t first, uint32_t last, uint32_t numFixed,
+                                                     const uint64_t* __restrict__ masks,
+                                                     const uint32_t* __restrict__ segOfs,
+                                                     uint32_t* __restrict__ bounds)
+{
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= numFixed) return;
+    uint32_t base = first + 64u * g;
+    uint32_t o    = segOfs[g];
+    uint64_t m    = masks[g];
+    bounds[o++]   = base;
+    while (m)
+    {
+        uint32_t b  = uint32_t(__ffsll((unsigned long long)m) - 1);
+        bounds[o++] = base + b + 1;
+        m &= m - 1;
+    }
+    if (g + 1 == numFixed) bounds[o] = last;
 }
 
 // ---------------------------------------------------------------------------
 // host driver shared by both entry points
 template<class Th, bool Sph>
 static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const double* y, const double* z, Th* h,
-                     const sphx_box* box, const sphx_octree* t, uint32_t ngmax, uint32_t ng0, ListLayout lay,
-                     uint32_t* nbrs, uint32_t* nc)
+                     const sphx_box* box, const sphx_octree* t, uint32_t ngmax, uint32_t ng0, uint32_t* nbrs,
+                     SphLists lists, uint32_t* nc)
 {
     if (G.numGroups == 0) return SPHX_OK;
-    uint32_t numG   = G.numGroups;
-    uint2*   ranges = scratchT<uint2>(ctx, S_NS_RANGES, size_t(numG) * NS_LCAP);
-    uint32_t* rcnt  = scratchT<uint32_t>(ctx, S_NS_RCOUNT, numG);
-    uint8_t* state  = scratchT<uint8_t>(ctx, S_NS_STATE, 2 * size_t(G.last));
-    uint32_t* redo  = scratchT<uint32_t>(ctx, S_NS_REDO, 2 * size_t(numG));
-    NsFlags* flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
-    if (!ranges || !rcnt || !state || !redo || !flags) return SPHX_ERR_NOMEM;
-    uint8_t* active = state;
-    uint8_t* iter   = state + G.last;
-    DBox     db     = makeDBox(box);
+    uint32_t  numG   = G.numGroups;
+    uint2*    ranges = scratchT<uint2>(ctx, S_NS_RANGES, size_t(numG) * NS_LCAP);
+    uint32_t* rcnt   = scratchT<uint32_t>(ctx, S_NS_RCOUNT, numG);
+    uint8_t*  iter   = scratchT<uint8_t>(ctx, S_NS_STATE, size_t(G.last));
+    uint32_t* redo   = scratchT<uint32_t>(ctx, S_NS_REDO, 2 * size_t(numG));
+    NsFlags*  flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
+    if (!ranges || !rcnt || !iter || !redo || !flags) return SPHX_ERR_NOMEM;
+    DBox db = makeDBox(box);
 
     if (G.last > G.first)
     {
-        nsInitState<<<gridFor(G.last - G.first, 256), 256, 0, ctx->stream>>>(active, iter, G.first, G.last);
+        nsInitState<<<gridFor(G.last - G.first, 256), 256, 0, ctx->stream>>>(iter, G.first, G.last);
         SPHX_LAUNCH_CHECK(ctx, "nsInitState");
     }
     nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 1);
     SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
+    ctx->nsTests = 0;
 
-    const uint32_t* list     = nullptr; // nullptr: all groups
+    const uint32_t* list      = nullptr; // nullptr: all groups
     uint32_t        numLaunch = numG;
-    int             pingpong = 0;
+    int             pingpong  = 0;
     for (int pass = 0; pass < 12 && numLaunch > 0; ++pass)
     {
         uint32_t* outList = redo + size_t(pingpong) * numG;
         {
             KTimer kt_(ctx, "nsGroupRanges");
-            nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, t->childOffsets,
-                                                            t->internalToLeaf, t->layout, t->centers, t->sizes, db,
-                                                            ranges, rcnt, flags);
+            nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, t->childOffsets,
+                                                                t->internalToLeaf, t->layout, t->centers, t->sizes,
+                                                                db, ranges, rcnt, flags);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsGroupRanges");
         {
             KTimer kt_(ctx, "nsCandidates");
-            nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, active, iter, ranges, rcnt,
-                                                                db, ngmax, ng0, lay, nbrs, nc, outList, flags);
+            nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, iter, ranges, rcnt, db,
+                                                                    ngmax, ng0, nbrs, lists, nc, outList, flags);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
         NsFlags hf;
         SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));
         SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->nsTests += double(hf.tests);
         if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
         if (hf.badGroup) return argError(ctx, "neighbor search: a target group holds more than 64 particles");
         if (!Sph) break;
@@ -471,6 +574,48 @@ using namespace sphx;
 extern "C"
 {
 
+int sphx_compute_groups(sphx_ctx* ctx, uint32_t first, uint32_t last, const double* x, const double* y,
+                        const double* z, const void* h, int32_t precision, const sphx_octree* tree,
+                        const sphx_box* box, float tolFactor, sphx_groups* groups)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!groups) return argError(ctx, "compute_groups: null output");
+    if (last < first) return argError(ctx, "compute_groups: last < first");
+    if (!validBox(box)) return argError(ctx, "compute_groups: invalid box");
+    if (!tree || tree->numLeafNodes < 1 || !tree->leaves || !tree->layout)
+        return argError(ctx, "compute_groups: tree needs numLeafNodes, leaves and layout");
+    if (precision != SPHX_FP64 && precision != SPHX_MIXED) return argError(ctx, "compute_groups: unknown precision");
+    uint32_t numFixed = (last - first + 63) / 64;
+    *groups           = sphx_groups{first, last, 0, nullptr, nullptr};
+    if (numFixed == 0) return SPHX_OK;
+    if (!x || !y || !z) return argError(ctx, "compute_groups: null coordinates");
+    uint64_t* masks = scratchT<uint64_t>(ctx, S_GROUPS_TMP, numFixed);
+    uint32_t* segs  = scratchT<uint32_t>(ctx, S_NS_RCOUNT, size_t(numFixed) + 1);
+    if (!masks || !segs) return SPHX_ERR_NOMEM;
+    DBox db = makeDBox(box);
+    if (precision == SPHX_FP64)
+        grpSplitMasks<double><<<numFixed, 64, 0, ctx->stream>>>(first, last, x, y, z, static_cast<const double*>(h),
+                                                                tree->leaves, tree->numLeafNodes, tree->layout, db,
+                                                                tolFactor, masks, segs);
+    else
+        grpSplitMasks<float><<<numFixed, 64, 0, ctx->stream>>>(first, last, x, y, z, static_cast<const float*>(h),
+                                                               tree->leaves, tree->numLeafNodes, tree->layout, db,
+                                                               tolFactor, masks, segs);
+    SPHX_LAUNCH_CHECK(ctx, "grpSplitMasks");
+    SPHX_TRY(exclusiveScanU32(ctx, segs, segs, numFixed));
+    uint32_t numGroups = 0;
+    SPHX_HIP(ctx, hipMemcpyAsync(&numGroups, segs + numFixed, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t* bounds = scratchT<uint32_t>(ctx, S_GROUPS, size_t(numGroups) + 1);
+    if (!bounds) return SPHX_ERR_NOMEM;
+    grpBoundaries<<<gridFor(numFixed, 256), 256, 0, ctx->stream>>>(first, last, numFixed, masks, segs, bounds);
+    SPHX_LAUNCH_CHECK(ctx, "grpBoundaries");
+    groups->numGroups  = numGroups;
+    groups->groupStart = bounds;
+    groups->groupEnd   = bounds + 1;
+    return SPHX_OK;
+}
+
 int sphx_find_neighbors(sphx_ctx* ctx, const double* x, const double* y, const double* z, const double* h,
                         uint32_t first, uint32_t last, const sphx_box* box, const sphx_octree* tree, uint32_t ngmax,
                         uint32_t* nbrs, uint32_t* nc)
@@ -483,8 +628,8 @@ int sphx_find_neighbors(sphx_ctx* ctx, const double* x, const double* y, const d
     if (!validTree(tree)) return argError(ctx, "find_neighbors: incomplete tree view");
     GroupsDev G;
     SPHX_TRY(groupsFromArgs(ctx, nullptr, first, last, G));
-    ListLayout lay{1, 1}; // CPU layout: [i - first][k]
-    return runSearch<double, false>(ctx, G, x, y, z, const_cast<double*>(h), box, tree, ngmax, 0, lay, nbrs, nc);
+    return runSearch<double, false>(ctx, G, x, y, z, const_cast<double*>(h), box, tree, ngmax, 0, nbrs, SphLists{},
+                                    nc);
 }
 
 int sphx_find_neighbors_sph(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box,
@@ -499,21 +644,26 @@ int sphx_find_neighbors_sph(sphx_ctx* ctx, const sphx_groups* groups, sphx_parti
     GroupsDev G;
     SPHX_TRY(groupsFromArgs(ctx, groups, 0, uint32_t(d->n), G));
     if (G.last > d->n) return argError(ctx, "find_neighbors_sph: group range exceeds n");
+    if (d->ngmax > 4096) return argError(ctx, "find_neighbors_sph: ngmax must be <= 4096");
     ctx->nsValid = false;
-    size_t    listLen = size_t(G.numGroups) * d->ngmax * 64;
-    uint32_t* lists   = scratchT<uint32_t>(ctx, S_NS_LISTS, listLen);
-    if (!lists) return SPHX_ERR_NOMEM;
-    ListLayout lay{64, 0}; // GPU layout: [group][k][lane]
-    int        st;
+    size_t   numSub = size_t(G.numGroups) * 4;
+    SphLists L;
+    L.capU   = 16u * d->ngmax;
+    L.slots  = scratchT<uint16_t>(ctx, S_NS_LISTS, numSub * d->ngmax * 16);
+    L.unions = scratchT<uint32_t>(ctx, S_NS_UNION, numSub * L.capU);
+    L.usize  = scratchT<uint32_t>(ctx, S_NS_USIZE, numSub);
+    if (!L.slots || !L.unions || !L.usize) return SPHX_ERR_NOMEM;
+    int st;
     if (d->precision == SPHX_FP64)
         st = runSearch<double, true>(ctx, G, d->x, d->y, d->z, static_cast<double*>(d->h), box, tree, d->ngmax,
-                                     d->ng0, lay, lists, d->nc);
+                                     d->ng0, nullptr, L, d->nc);
     else if (d->precision == SPHX_MIXED)
         st = runSearch<float, true>(ctx, G, d->x, d->y, d->z, static_cast<float*>(d->h), box, tree, d->ngmax,
-                                    d->ng0, lay, lists, d->nc);
+                                    d->ng0, nullptr, L, d->nc);
     else
         return argError(ctx, "find_neighbors_sph: unknown precision");
     if (st != SPHX_OK) return st;
+    ctx->nsLists  = L;
     ctx->nsValid  = true;
     ctx->nsGroups = G;
     ctx->nsNgmax  = d->ngmax;
@@ -528,8 +678,7 @@ int sphx_export_neighbors(sphx_ctx* ctx, uint32_t* nbrs)
     if (!nbrs) return argError(ctx, "export_neighbors: null pointer");
     const GroupsDev& G = ctx->nsGroups;
     if (G.numGroups == 0) return SPHX_OK;
-    const uint32_t* lists = static_cast<const uint32_t*>(ctx->buf[S_NS_LISTS].p);
-    nsExport<<<gridFor(G.numGroups, 4), 256, 0, ctx->stream>>>(G, lists, ctx->nsNgmax, ctx->nsNc, nbrs);
+    nsExport<<<gridFor(G.numGroups, 4), 256, 0, ctx->stream>>>(G, ctx->nsLists, ctx->nsNgmax, ctx->nsNc, nbrs);
     SPHX_LAUNCH_CHECK(ctx, "nsExport");
     return SPHX_OK;
 }

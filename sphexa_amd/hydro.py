@@ -97,6 +97,17 @@ class Context:
         self.check(self.lib.sphx_timing_summary(self.h, cap, names, launches, ms, C.byref(num)))
         return {names[k].decode(): (launches[k], ms[k]) for k in range(min(num.value, cap))}
 
+    def set_loop_shape(self, precision: int, shape: int):
+        """Particle-loop kernel shape: 0 LDS-staged sub-groups, 1 one lane per target."""
+        opt = _lib.OPT_LOOP_SHAPE_FP64 if precision == FP64 else _lib.OPT_LOOP_SHAPE_MIXED
+        self.check(self.lib.sphx_ctx_set_option(self.h, opt, int(shape)))
+
+    def candidate_tests(self) -> float:
+        """(target, candidate) pairs tested by the last neighbor search."""
+        v = C.c_double()
+        self.check(self.lib.sphx_get_stat(self.h, _lib.STAT_CANDIDATE_TESTS, C.byref(v)))
+        return v.value
+
     def synchronize(self):
         self.check(self.lib.sphx_ctx_synchronize(self.h))
 
@@ -320,9 +331,45 @@ class ParticlesData:
         return s
 
 
+@dataclass
+class TargetGroups:
+    """Target groups over [first, last): group g is [bounds[g], bounds[g+1]) (cstone::GroupData,
+    domain/include/cstone/traversal/groups.hpp)."""
+
+    first: int
+    last: int
+    numGroups: int
+    bounds: torch.Tensor  # int32[numGroups + 1]
+
+    def struct(self) -> Groups:
+        p = _ptr(self.bounds)
+        return Groups(self.first, self.last, self.numGroups, p, p + 4)
+
+
+def compute_groups(ctx: Context, x, y, z, h, octree: OctreeData, box: Box, tol_factor: float = 2.0, first=0,
+                   last=None) -> TargetGroups:
+    """cstone::computeGroupSplits with 64-particle groups (traversal/groups_gpu.cu:108-137), as
+    called by sph::computeSpatialGroups with tolFactor 2 (sph/groups.hpp). h may be None
+    (the spatial criterion alone); its dtype selects FP64 or mixed fields."""
+    last = x.numel() if last is None else last
+    prec = MIXED if (h is not None and h.dtype == torch.float32) else FP64
+    v = octree.view()
+    g = Groups()
+    ctx.check(ctx.lib.sphx_compute_groups(ctx.h, first, last, _ptr(x), _ptr(y), _ptr(z), _ptr(h), prec, C.byref(v),
+                                          C.byref(box), tol_factor, C.byref(g)))
+    bounds = torch.empty(g.numGroups + 1, dtype=torch.int32, device=x.device)
+    if g.numGroups > 0:
+        ctx.d2d(bounds, g.groupStart, 4 * (g.numGroups + 1))
+    else:
+        bounds.fill_(last)
+    return TargetGroups(first, last, g.numGroups, bounds)
+
+
 def _groups_struct(groups, d: ParticlesData):
     if groups is None:
         return Groups(0, d.n, (d.n + 63) // 64, None, None)
+    if isinstance(groups, TargetGroups):
+        return groups.struct()
     return groups
 
 
@@ -364,7 +411,7 @@ def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups
 
 
 # ------------------------------------------------------------------------------------------- the step
-STAGES = ("keys", "sort", "gather", "octree", "link", "centers", "neighbors", "density", "eos", "iad",
+STAGES = ("keys", "sort", "gather", "octree", "link", "centers", "groups", "neighbors", "density", "eos", "iad",
           "momentum_energy")
 
 
@@ -375,12 +422,16 @@ class HydroStep:
 
     ``compute_forces(inputs)`` takes unsorted device tensors x, y, z (float64) and
     h, m, vx, vy, vz, u (precision dtype), and leaves the SFC-sorted results in
-    ``self.d`` (a ParticlesData) and the tree in ``self.octree``.
+    ``self.d`` (a ParticlesData), the tree in ``self.octree`` and the target groups in
+    ``self.groups``.  ``spatial_groups`` selects computeSpatialGroups' split groups (the
+    reference's GPU path, sph/groups.hpp:42-67) over plain 64-particle groups.
     """
 
     def __init__(self, ctx: Context, n: int, box: Box, precision=FP64, bucket_size=64, ng0=100, ngmax=150,
-                 sinc_index=6.0, gamma=5.0 / 3.0, Kcour=0.2):
+                 sinc_index=6.0, gamma=5.0 / 3.0, Kcour=0.2, spatial_groups=True):
         self.ctx, self.n, self.box, self.bucket = ctx, n, box, bucket_size
+        self.spatial_groups = spatial_groups
+        self.groups = None
         self.d = ParticlesData(n, precision, ng0, ngmax, Kcour, gamma, sinc_index).allocate(ctx.device)
         self.keys = torch.empty(n, dtype=torch.int64, device=ctx.device)
         self.order = torch.empty(n, dtype=torch.int32, device=ctx.device)
@@ -409,16 +460,19 @@ class HydroStep:
         node_fp_centers(ctx, o, box)
         self.octree = o
         mark(6)
-        find_neighbors_sfc(ctx, d, box, o)
+        g = compute_groups(ctx, d.x, d.y, d.z, d.h, o, box) if self.spatial_groups else None
+        self.groups = g
         mark(7)
-        compute_density(ctx, d, box)
+        find_neighbors_sfc(ctx, d, box, o, g)
         mark(8)
-        compute_eos_hydro_std(ctx, d)
+        compute_density(ctx, d, box, g)
         mark(9)
-        compute_iad(ctx, d, box)
+        compute_eos_hydro_std(ctx, d)
         mark(10)
-        compute_momentum_energy_std(ctx, d, box)
+        compute_iad(ctx, d, box, g)
         mark(11)
+        compute_momentum_energy_std(ctx, d, box, g)
+        mark(12)
         self.events = ev
         return d
 

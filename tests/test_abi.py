@@ -1,0 +1,141 @@
+"""CPU checks of the drop-in boundary: libsphx.so loads and exports every function that
+include/sphx.h declares, the host-side tables equal the oracle's bit for bit, argument
+errors come back as status codes (no GPU call), and include/sphx_compat.hpp compiles
+against mock types and against the reference's own types."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from sphexa_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sphx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(sphx_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "libsphx.so not built"
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    unbound = [n for n in names if n not in _lib.SIGNATURES]
+    assert not unbound, f"declared but missing from the ctypes table: {unbound}"
+
+
+def test_kernel_tables_match_oracle():
+    from sphexa_amd.hydro import kernel_tables
+
+    wh64, wh32, K = kernel_tables(6.0)
+    o64, o32, oK = oracle.Cpu("oracle").kernel_tables(6.0)
+    assert np.array_equal(wh64, o64) and np.array_equal(wh32, o32) and K == oK
+
+
+def test_octree_sizes():
+    lib = _lib.load()
+    ni, nn, npar = C.c_int32(), C.c_int32(), C.c_int32()
+    assert lib.sphx_octree_sizes(15, C.byref(ni), C.byref(nn), C.byref(npar)) == 0
+    assert (ni.value, nn.value, npar.value) == (2, 17, 2)
+    assert lib.sphx_octree_sizes(1, C.byref(ni), C.byref(nn), C.byref(npar)) == 0
+    assert (ni.value, nn.value, npar.value) == (0, 1, 1)
+    assert lib.sphx_octree_sizes(0, None, None, None) == _lib.SPHX_ERR_ARG
+
+
+def test_null_context_is_an_argument_error():
+    lib = _lib.load()
+    assert lib.sphx_compute_sfc_keys(None, None, None, None, None, 10, None) == _lib.SPHX_ERR_ARG
+    assert lib.sphx_sort_by_key(None, None, None, 10) == _lib.SPHX_ERR_ARG
+    assert lib.sphx_ctx_destroy(None) == 0
+    assert lib.sphx_last_error(None) == b"null context"
+    assert lib.sphx_version().startswith(b"sphx")
+
+
+def test_context_without_device():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.sphx_ctx_create(0, None, C.byref(h)) == _lib.SPHX_ERR_NODEV
+
+
+MOCK = r"""
+#include <cstddef>
+#include <vector>
+#include "sphx_compat.hpp"
+enum class BT : char { open = 0, periodic = 1 };
+struct MockBox {
+    double xmin() const { return 0; } double xmax() const { return 1; }
+    double ymin() const { return 0; } double ymax() const { return 1; }
+    double zmin() const { return 0; } double zmax() const { return 1; }
+    BT boundaryX() const { return BT::periodic; } BT boundaryY() const { return BT::periodic; }
+    BT boundaryZ() const { return BT::periodic; }
+};
+template<class T> struct DevVec { T* p = nullptr; size_t n = 0; T* data() { return p; } size_t size() const { return n; } };
+struct Data {
+    DevVec<double> x, y, z; DevVec<float> h, m, vx, vy, vz, u, rho, p, c, c11, c12, c13, c22, c23, c33, ax, ay, az, du, wh;
+    DevVec<unsigned> nc; unsigned ng0 = 100, ngmax = 150; double K = 0.79, gamma = 5.0 / 3.0, minDtCourant = 0; float Kcour = 0.2f;
+};
+struct Group { unsigned firstBody, lastBody, numGroups; const unsigned* groupStart; const unsigned* groupEnd; };
+void use(Data& d, const Group& g) {
+    MockBox box;
+    sphx_compat::computeDensity(g, d, box);
+    sphx_compat::computeEOS_HydroStd(0, 10, d);
+    sphx_compat::computeIAD(g, d, box);
+    sphx_compat::computeMomentumEnergySTD(g, d, box);
+    sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
+    (void)sphx_compat::computeGroupSplits(0u, 0u, nullptr, nullptr, nullptr, (const double*)nullptr, nullptr, 0,
+                                          nullptr, box);
+}
+"""
+
+REFERENCE_TYPES = r"""
+#include <utility>
+#include "cstone/findneighbors.hpp"
+#include "cstone/traversal/groups.hpp"
+#include "sphx_compat.hpp"
+void use(const cstone::OctreeNsView<double, uint64_t>& view, const cstone::GroupView& g) {
+    cstone::Box<double> box(0, 1, cstone::BoundaryType::periodic);
+    sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
+    sphx_compat::findNeighbors(nullptr, nullptr, nullptr, nullptr, 0, 0, box, view, 0, 150, nullptr, nullptr);
+    sphx_compat::nodeFpCenters<cstone::Vec3<double>>(nullptr, 0, nullptr, nullptr, box);
+    (void)sphx_compat::toGroups(g);
+    auto s = sphx_compat::computeGroupSplits(0u, 0u, nullptr, nullptr, nullptr, (const float*)nullptr, nullptr, 0,
+                                             nullptr, box);
+    (void)sphx_compat::toGroups(s);
+}
+"""
+
+
+def _syntax_check(src, extra_includes=()):
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "t.cpp")
+        open(f, "w").write(src)
+        cmd = ["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include")]
+        for inc in extra_includes:
+            cmd += ["-I", inc]
+        out = subprocess.run(cmd + [f], capture_output=True, text=True)
+        assert out.returncode == 0, out.stderr[-3000:]
+
+
+def test_compat_header_compiles_with_mock_types():
+    _syntax_check(MOCK)
+
+
+def test_compat_header_compiles_with_reference_types():
+    ref = "/root/reference/domain/include"
+    if not os.path.isdir(ref):
+        pytest.skip("reference sources not present")
+    _syntax_check(REFERENCE_TYPES, [ref])

@@ -278,3 +278,66 @@ def test_export_neighbors_cpu_layout(ctx):
     for i in range(d.n):
         c = min(int(nc[i]) - 1, d.ngmax)
         assert np.array_equal(nb[i, :c], ref["nbrs"][i, :c]), i
+
+
+# ----------------------------------------------------------------------------- target groups
+@pytest.mark.parametrize("case", ["uniform", "sedov", "clustered"])
+@pytest.mark.parametrize("hdtype", ["f64", "f32", "none"])
+def test_compute_groups_bit_exact(ctx, case, hdtype):
+    """computeGroupSplits on the GPU against the oracle: identical boundaries."""
+    from sphexa_amd import hydro
+
+    print("case groups", case, hdtype, flush=True)
+    if case == "sedov":
+        inp, box = sedov_lattice(24)
+    elif case == "uniform":
+        inp, box = uniform_box(30000, seed=51)
+    else:
+        # two dense clumps in a sparse background: large SFC jumps between them
+        inp, box = uniform_box(20000, seed=52, periodic=False)
+        rng = np.random.default_rng(53)
+        for k, c in ((0, 0.2), (1, 0.7)):
+            sl = slice(k * 6000, (k + 1) * 6000)
+            for a in ("x", "y", "z"):
+                inp[a][sl] = c + 0.02 * rng.random(6000)
+    keys, s, t = _gpu_tree(ctx, inp, box)
+    rt = CPU.tree(keys, box)
+    n = len(s["x"])
+    if hdtype == "none":
+        h_dev, h_ref = None, None
+    elif hdtype == "f32":
+        h_dev = dev(s["h"], torch.float32)
+        h_ref = s["h"].astype(np.float32).astype(np.float64)
+    else:
+        h_dev, h_ref = dev(s["h"]), s["h"]
+    for first, last in ((0, n), (37, n - 101)):
+        ref = CPU.compute_groups(s["x"], s["y"], s["z"], h_ref, rt["leaves"], rt["layout"], box, 2.0, first, last)
+        g = hydro.compute_groups(ctx, dev(s["x"]), dev(s["y"]), dev(s["z"]), h_dev, t, hbox(box), 2.0, first, last)
+        got = u32(g.bounds)
+        assert g.numGroups == len(ref) - 1
+        assert np.array_equal(got, ref)
+        sizes = np.diff(got.astype(np.int64))
+        assert sizes.min() >= 1 and sizes.max() <= 64
+
+
+@pytest.mark.parametrize("case", ["uniform_periodic", "sedov"])
+def test_hydro_step_spatial_groups_equal_fixed(ctx, case):
+    """Results are per particle: split groups give the same bits as fixed 64-particle groups
+    in the FP64-exact loops, and the same neighbor counts."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64
+
+    print("case spatial groups", case, flush=True)
+    inp, box = sedov_lattice(24) if case == "sedov" else uniform_box(20000, seed=61)
+    out = {}
+    for spatial in (False, True):
+        ctx.set_fp64_exact(True)
+        step = hydro.HydroStep(ctx, len(inp["x"]), hbox(box), FP64, spatial_groups=spatial)
+        d = step.compute_forces({k: dev(v) for k, v in inp.items()})
+        torch.cuda.synchronize()
+        ctx.set_fp64_exact(False)
+        out[spatial] = {f: getattr(d, f).cpu().numpy().copy() for f in FIELDS + ("nc", "h")}
+        if spatial:
+            assert step.groups is not None and step.groups.numGroups >= (len(inp["x"]) + 63) // 64
+    for f in FIELDS + ("nc", "h"):
+        assert np.array_equal(out[True][f], out[False][f]), f

@@ -212,10 +212,13 @@ int sphx_node_centers(sphx_ctx* ctx, const uint64_t* prefixes, int32_t numNodes,
 /* ---- target groups -------------------------------------------------------- */
 /* sph::computeGroups -> cstone::computeGroupSplits (sph/include/sph/groups.hpp:41-74,
  * traversal/groups_gpu.cu:108-137): SFC-consecutive groups of at most 64
- * particles, split where consecutive particles are farther apart than
- * min(tolFactor * cbrt(smallest leaf volume in the group), h).  Writes group
- * boundaries into context memory; *groups is filled with device pointers valid
- * until the next call. */
+ * particles: each fixed 64-particle group is split after particle p when
+ * |X(p+1) - X(p)| > min(tolFactor * cbrt(smallest leaf volume in the group),
+ * 2 h_p / min box length), X = position / box lengths (unit-box coordinates; h may be
+ * NULL: 2h term = 1; h is double for SPHX_FP64, float for SPHX_MIXED).  Writes
+ * group boundaries into context memory; *groups is filled with device pointers
+ * valid until the next call (groupEnd = groupStart + 1).  Synchronises the stream
+ * once (to return numGroups).  tree needs numLeafNodes, leaves and layout. */
 int sphx_compute_groups(sphx_ctx* ctx, uint32_t first, uint32_t last, const double* x, const double* y,
                         const double* z, const void* h, int32_t precision, const sphx_octree* tree,
                         const sphx_box* box, float tolFactor, sphx_groups* groups);
@@ -260,6 +263,27 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
  * the stream once to return minDtCourant. */
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                  const sphx_box* box);
+
+/* ---- adjoints -------------------------------------------------------------- */
+/* No counterpart in the reference (its loops are forward-only).  Vector-Jacobian
+ * product of sphx_compute_density with respect to the positions and h, on the lists
+ * of the last sphx_find_neighbors_sph (same groups, covering [0, n)): given
+ * gRho[i] = dL/drho_i (field type), writes gx[i] = dL/dx_i, gy, gz (double, n each,
+ * overwritten) and, if gh != NULL, gh[i] = dL/dh_i (field type).  The derivative is
+ * that of the forward's own arithmetic: the table-linear kernel (FP64) or the FP32
+ * polynomial kernel (MIXED); neighbor sets are held fixed.  j-side terms are summed
+ * with atomic adds: reproducible to rounding, not bit for bit. */
+int sphx_density_adjoint(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d, const sphx_box* box,
+                         const void* gRho, double* gx, double* gy, double* gz, void* gh);
+
+/* No counterpart in the reference.  Vector-Jacobian product of
+ * sphx_compute_momentum_energy with respect to the positions, holding v, h, m, rho,
+ * p, c and c11..c33 fixed: given dL/dax, dL/day, dL/daz, dL/ddu (field type; any may
+ * be NULL = zero), writes gx, gy, gz (double, n each, overwritten).  Lists and groups
+ * as for sphx_density_adjoint. */
+int sphx_momentum_energy_adjoint(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d,
+                                 const sphx_box* box, const void* gax, const void* gay, const void* gaz,
+                                 const void* gdu, double* gx, double* gy, double* gz);
 
 #ifdef __cplusplus
 }

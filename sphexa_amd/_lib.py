@@ -87,6 +87,10 @@ SIGNATURES = {
     "sphx_compute_eos": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles)]),
     "sphx_compute_iad": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_momentum_energy": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
+    "sphx_density_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V, _V, _V,
+                                       _V, _V]),
+    "sphx_momentum_energy_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V,
+                                               _V, _V, _V, _V, _V, _V]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),
                                       C.POINTER(C.c_int)]),

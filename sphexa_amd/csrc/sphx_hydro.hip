@@ -10,6 +10,7 @@
 #define SPHX_HYDRO_NS fast
 #define SPHX_HYDRO_EXACT 0
 #include "sphx_hydro_kernels.inc"
+#include "sphx_adjoint.inc"
 
 namespace sphx
 {
@@ -239,6 +240,45 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     std::memcpy(&v, &bits, sizeof(v));
     d->minDtCourant = v;
     return SPHX_OK;
+}
+
+int sphx_density_adjoint(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d, const sphx_box* box,
+                         const void* gRho, double* gx, double* gy, double* gz, void* gh)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "density_adjoint: null field"));
+    if (!gRho || !gx || !gy || !gz) return argError(ctx, "density_adjoint: null gradient pointer");
+    if (!validBox(box)) return argError(ctx, "density_adjoint: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    if (G.first != 0 || G.last != d->n) return argError(ctx, "density_adjoint: groups must cover [0, n)");
+    DBox db = makeDBox(box);
+    return d->precision == SPHX_FP64 ? fast::densityAdjointT<double>(ctx, G, lists, d, db, gRho, gx, gy, gz, gh)
+                                     : fast::densityAdjointT<float>(ctx, G, lists, d, db, gRho, gx, gy, gz, gh);
+}
+
+int sphx_momentum_energy_adjoint(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d,
+                                 const sphx_box* box, const void* gax, const void* gay, const void* gaz,
+                                 const void* gdu, double* gx, double* gy, double* gz)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "momentum_energy_adjoint: null field"));
+    if (!d->vx || !d->vy || !d->vz || !d->rho || !d->p || !d->c || !d->c11 || !d->c12 || !d->c13 || !d->c22 ||
+        !d->c23 || !d->c33)
+        return argError(ctx, "momentum_energy_adjoint: null field");
+    if (!gx || !gy || !gz) return argError(ctx, "momentum_energy_adjoint: null gradient pointer");
+    if (!validBox(box)) return argError(ctx, "momentum_energy_adjoint: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    if (G.first != 0 || G.last != d->n) return argError(ctx, "momentum_energy_adjoint: groups must cover [0, n)");
+    DBox db = makeDBox(box);
+    return d->precision == SPHX_FP64
+               ? fast::meAdjointT<double>(ctx, G, lists, d, db, gax, gay, gaz, gdu, gx, gy, gz)
+               : fast::meAdjointT<float>(ctx, G, lists, d, db, gax, gay, gaz, gdu, gx, gy, gz);
 }
 
 } // extern "C"

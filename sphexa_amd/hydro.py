@@ -410,6 +410,35 @@ def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups
     return d.minDtCourant
 
 
+# ------------------------------------------------------------------------------------------- adjoints
+def density_adjoint(ctx: Context, d: ParticlesData, box: Box, g_rho, groups=None, want_h=True):
+    """Adjoint of compute_density on the context's lists: (dL/dx, dL/dy, dL/dz[, dL/dh]) for
+    g_rho = dL/drho (no counterpart in the reference)."""
+    dev = d.x.device
+    g_rho = g_rho.to(d.dtype).contiguous()
+    gx, gy, gz = (torch.empty(d.n, dtype=torch.float64, device=dev) for _ in range(3))
+    gh = torch.empty(d.n, dtype=d.dtype, device=dev) if want_h else None
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_density_adjoint(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box),
+                                           _ptr(g_rho), _ptr(gx), _ptr(gy), _ptr(gz), _ptr(gh)))
+    return (gx, gy, gz, gh) if want_h else (gx, gy, gz)
+
+
+def momentum_energy_adjoint(ctx: Context, d: ParticlesData, box: Box, g_ax=None, g_ay=None, g_az=None, g_du=None,
+                            groups=None):
+    """Adjoint of compute_momentum_energy_std with respect to x, y, z (v, h, m, rho, p, c and
+    the IAD tensors held fixed): (dL/dx, dL/dy, dL/dz) for the given output gradients."""
+    dev = d.x.device
+    conv = lambda t: None if t is None else t.to(d.dtype).contiguous()
+    g_ax, g_ay, g_az, g_du = conv(g_ax), conv(g_ay), conv(g_az), conv(g_du)
+    gx, gy, gz = (torch.empty(d.n, dtype=torch.float64, device=dev) for _ in range(3))
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_momentum_energy_adjoint(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
+                                                   C.byref(box), _ptr(g_ax), _ptr(g_ay), _ptr(g_az), _ptr(g_du),
+                                                   _ptr(gx), _ptr(gy), _ptr(gz)))
+    return gx, gy, gz
+
+
 # ------------------------------------------------------------------------------------------- the step
 STAGES = ("keys", "sort", "gather", "octree", "link", "centers", "groups", "neighbors", "density", "eos", "iad",
           "momentum_energy")
@@ -475,6 +504,16 @@ class HydroStep:
         mark(12)
         self.events = ev
         return d
+
+    def momentum_energy_vjp(self, g_ax=None, g_ay=None, g_az=None, g_du=None, recompute_lists=True):
+        """dL/d(x, y, z), in SFC order, of the last compute_forces' momentum-energy outputs for the
+        given output gradients (rho, p, c and the IAD tensors held fixed).  recompute_lists re-runs
+        the neighbor search from the final smoothing lengths first, as a backward pass that does
+        not keep the forward's lists would; it reproduces the forward's lists unless a particle
+        ended the forward's h iteration unconverged."""
+        if recompute_lists:
+            find_neighbors_sfc(self.ctx, self.d, self.box, self.octree, self.groups)
+        return momentum_energy_adjoint(self.ctx, self.d, self.box, g_ax, g_ay, g_az, g_du, self.groups)
 
     def stage_times_ms(self):
         """Per-stage milliseconds of the last timed compute_forces (synchronises)."""

@@ -1,0 +1,140 @@
+"""GPU adjoints of density and momentum-energy against the FP64 torch reference
+(tests/adjoint_ref.py, itself pinned to the oracle by value and by central differences).
+
+Inputs are the GPU forward's own sorted fields and exported lists, so the comparison
+isolates the adjoint kernels.  FP64 tolerance 1e-11 of the gradient's max (atomic
+summation order); MIXED 2e-3 (FP32 pair arithmetic and the polynomial kernel's exact
+derivative against the table's segment slopes)."""
+import numpy as np
+import pytest
+import torch
+
+from tests import adjoint_ref as R
+from tests.helpers import uniform_box
+
+pytestmark = pytest.mark.gpu
+
+ME_F = ("vx", "vy", "vz", "h", "m", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from sphexa_amd import hydro
+
+    c = hydro.Context(0)
+    yield c
+    c.close()
+
+
+def _forward(ctx, n, seed, precision, periodic=True):
+    from sphexa_amd import hydro
+
+    inp, box = uniform_box(n, seed=seed, periodic=periodic)
+    hb = hydro.make_box(box.xmin, box.xmax, box.bx)
+    step = hydro.HydroStep(ctx, n, hb, precision)
+    dt = step.d.dtype
+    d = step.compute_forces({k: torch.tensor(v, dtype=torch.float64 if k in ("x", "y", "z") else dt, device=ctx.device)
+                             for k, v in inp.items()})
+    torch.cuda.synchronize()
+    nb = hydro.export_neighbors(ctx, d).cpu().numpy().view(np.uint32)
+    nc = d.nc.cpu().numpy().view(np.uint32)
+    i, j = R.pairs_from_lists(nb, nc, d.ngmax)
+    wh = torch.from_numpy(hydro.kernel_tables(6.0)[0])
+    return step, d, hb, box, i, j, wh
+
+
+def _err(got, ref):
+    return float((got.cpu().double() - ref).abs().max() / ref.abs().max())
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_density_adjoint(ctx, mode, periodic):
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64, MIXED
+
+    print("case density_adjoint", mode, periodic, flush=True)
+    step, d, hb, box, i, j, wh = _forward(ctx, 12000, 81, FP64 if mode == "fp64" else MIXED, periodic)
+    g = torch.from_numpy(np.random.default_rng(82).normal(size=d.n))
+    gx, gy, gz, gh = hydro.density_adjoint(ctx, d, hb, g.to(ctx.device), step.groups)
+    xyz = [d.x.cpu().double().requires_grad_(True), d.y.cpu().double().requires_grad_(True),
+           d.z.cpu().double().requires_grad_(True)]
+    h = d.h.cpu().double().requires_grad_(True)
+    rho = R.density(*xyz, h, d.m.cpu().double(), i, j, wh, d.K, box)
+    ref = torch.autograd.grad((rho * g).sum(), xyz + [h])
+    tol = 1e-11 if mode == "fp64" else 2e-3
+    errs = [_err(a, b) for a, b in zip((gx, gy, gz, gh), ref)]
+    print("errors", errs, flush=True)
+    assert max(errs) < tol
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_momentum_energy_adjoint(ctx, mode):
+    from sphexa_amd._lib import FP64, MIXED
+
+    print("case me_adjoint", mode, flush=True)
+    step, d, hb, box, i, j, wh = _forward(ctx, 12000, 83, FP64 if mode == "fp64" else MIXED)
+    W = torch.from_numpy(np.random.default_rng(84).normal(size=(4, d.n)))
+    gx, gy, gz = step.momentum_energy_vjp(*[W[k].to(ctx.device) for k in range(4)])
+    f = {k: getattr(d, k).cpu().double() for k in ME_F}
+    xyz = [d.x.cpu().double().requires_grad_(True), d.y.cpu().double().requires_grad_(True),
+           d.z.cpu().double().requires_grad_(True)]
+    out = R.momentum_energy(*xyz, f, i, j, wh, d.K, box)
+    ref = torch.autograd.grad(sum((W[k] * out[k]).sum() for k in range(4)), xyz)
+    tol = 1e-11 if mode == "fp64" else 2e-3
+    errs = [_err(a, b) for a, b in zip((gx, gy, gz), ref)]
+    print("errors", errs, flush=True)
+    assert max(errs) < tol
+    # one output gradient alone (the others NULL) is the matching part of the sum
+    gx1, _, _ = step.momentum_energy_vjp(None, None, None, W[3].to(ctx.device), recompute_lists=False)
+    ref1 = torch.autograd.grad((W[3] * R.momentum_energy(*xyz, f, i, j, wh, d.K, box)[3]).sum(), xyz)
+    assert _err(gx1, ref1[0]) < tol
+
+
+def test_density_op_autograd_unsorted_inputs(ctx):
+    """DensityOp: caller-order inputs, autograd gradients equal the sorted adjoint mapped back."""
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+
+    print("case density_op", flush=True)
+    inp, box = uniform_box(8000, seed=85)
+    hb = hydro.make_box(box.xmin, box.xmax, box.bx)
+    op = autodiff.DensityOp(ctx, hb, FP64, ngmax=256)
+    t = {k: torch.tensor(inp[k], device=ctx.device) for k in ("x", "y", "z", "h", "m")}
+    xyz = [t[k].clone().requires_grad_(True) for k in "xyz"]
+    h = t["h"].clone().requires_grad_(True)
+    rho = op(*xyz, h, t["m"])
+    loss = ((rho - rho.mean()) ** 2).sum()
+    grads = torch.autograd.grad(loss, xyz + [h])
+    # reference in the caller's order: pairs from the oracle-free exported lists of the op
+    d, order = op.pipe.d, op.pipe.order.long().cpu()
+    nb = hydro.export_neighbors(ctx, d).cpu().numpy().view(np.uint32)
+    i, j = R.pairs_from_lists(nb, d.nc.cpu().numpy().view(np.uint32), d.ngmax)
+    wh = torch.from_numpy(hydro.kernel_tables(6.0)[0])
+    xs = [torch.tensor(inp[k])[order].requires_grad_(True) for k in "xyz"]
+    hs = torch.tensor(inp["h"])[order].requires_grad_(True)
+    rr = R.density(*xs, hs, torch.tensor(inp["m"])[order], i, j, wh, d.K, box)
+    assert float((rr - rho.detach().cpu()[order]).abs().max() / rr.abs().max()) < 1e-13
+    ref = torch.autograd.grad(((rr - rr.mean()) ** 2).sum(), xs + [hs])
+    for g, r in zip(grads, ref):
+        assert _err(g.cpu()[order], r) < 1e-11
+
+
+def test_glass_relaxation_reduces_density_variance(ctx):
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+    from sphexa_amd.workloads import jittered_lattice
+
+    print("case relax_glass", flush=True)
+    side = 16
+    f = jittered_lattice(side, 0.3, seed=86)[0]
+    x, y, z = f["x"], f["y"], f["z"]
+    n = side ** 3
+    hb = hydro.make_box(0.0, 1.0, 1)
+    dev = ctx.device
+    h = torch.full((n,), 0.5 * (300 / (4 * np.pi * n)) ** (1 / 3), dtype=torch.float64, device=dev)
+    m = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+    X = [torch.tensor(a, device=dev) for a in (x, y, z)]
+    _, _, _, hist = autodiff.relax_glass(ctx, hb, *X, h, m, iterations=20, precision=FP64)
+    print("loss", hist[0], hist[-1], flush=True)
+    assert hist[-1] < 0.5 * hist[0]

@@ -341,3 +341,18 @@ def test_hydro_step_spatial_groups_equal_fixed(ctx, case):
             assert step.groups is not None and step.groups.numGroups >= (len(inp["x"]) + 63) // 64
     for f in FIELDS + ("nc", "h"):
         assert np.array_equal(out[True][f], out[False][f]), f
+
+
+def test_compute_groups_reference_find_splits_known_answer(ctx):
+    """The reference's findSplits known answer on the GPU (see tests/test_oracle.py)."""
+    from sphexa_amd import hydro
+    from tests.test_oracle import _find_splits_case
+
+    print("case groups find_splits", flush=True)
+    p, h, tol = _find_splits_case()
+    leaves = dev(np.array([0, 1 << 63], dtype=np.uint64).view(np.int64))
+    layout = dev(np.array([0, 64], dtype=np.int32))
+    t = hydro.build_octree(ctx, leaves)
+    t.layout = layout
+    g = hydro.compute_groups(ctx, dev(p), dev(p), dev(p), dev(h), t, hydro.make_box(0.0, 1.0, 0), tol)
+    assert u32(g.bounds).tolist() == [0, 1, 3, 32, 34, 64]

@@ -440,3 +440,25 @@ def test_groups_box_scaling_and_ragged_range():
     assert b.tolist() == [5, 30, 69, 133, 150]
     assert CPU.compute_groups(x, y, z, h, _leaves_uniform(0), _layout_all_in_leaf0(n, 1), Box.cube(0.0, 2.0),
                               first=7, last=7).tolist() == [7]
+
+
+def _find_splits_case():
+    """The reference's findSplits known answer (domain/test/unit_cuda/traversal/groups.cu:51-112)
+    through the group interface: x = y = z = lane, distCrit^2 = 3.01 (tolFactor = sqrt(3.01) on
+    a root leaf), 2h = 64 except lane 2 (2h = 0.99 sqrt 3); lane 0 moved to -1, lanes 31 and
+    33 moved by -0.5.  Splits after lanes 0, 2, 31 and 33."""
+    n = 64
+    p = np.arange(n, dtype=np.float64)
+    p[0] = -1.0
+    p[31] -= 0.5
+    p[33] -= 0.5
+    h = np.full(n, 32.0)
+    h[2] = 0.5 * 0.99 * np.sqrt(3.0)
+    return p, h, float(np.float32(np.sqrt(3.01)))
+
+
+def test_groups_reference_find_splits_known_answer():
+    p, h, tol = _find_splits_case()
+    b = CPU.compute_groups(p, p.copy(), p.copy(), h, _leaves_uniform(0), _layout_all_in_leaf0(64, 1),
+                           Box.cube(0.0, 1.0), tol=tol)
+    assert b.tolist() == [0, 1, 3, 32, 34, 64]

@@ -342,7 +342,7 @@ class TargetGroups:
     bounds: torch.Tensor  # int32[numGroups + 1]
 
     def struct(self) -> Groups:
-        p = _ptr(self.bounds)
+        p = self.bounds.data_ptr()
         return Groups(self.first, self.last, self.numGroups, p, p + 4)
 
 

@@ -139,3 +139,14 @@ def test_compat_header_compiles_with_reference_types():
     if not os.path.isdir(ref):
         pytest.skip("reference sources not present")
     _syntax_check(REFERENCE_TYPES, [ref])
+
+
+def test_target_groups_struct_points_into_bounds():
+    import torch
+
+    from sphexa_amd.hydro import TargetGroups
+
+    b = torch.tensor([0, 64, 100], dtype=torch.int32)
+    g = TargetGroups(0, 100, 2, b).struct()
+    assert (g.firstBody, g.lastBody, g.numGroups) == (0, 100, 2)
+    assert g.groupStart == b.data_ptr() and g.groupEnd == b.data_ptr() + 4

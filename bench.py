@@ -87,13 +87,42 @@ def roofline(kernels, pairs, n, steps, precision, tests):
     else:
         flop = None
         unit_note = "no algorithmic flop count"
-    peak = PEAK_FP64_TFLOPS if precision == 0 else PEAK_FP32_TFLOPS
+    # the neighbor predicate is FP64 in both precisions; the particle loops compute in
+    # the precision of the fields
+    fp64 = precision == 0 or name.startswith("ns")
+    peak = PEAK_FP64_TFLOPS if fp64 else PEAK_FP32_TFLOPS
     achieved = (flop / (per_launch_ms * 1e-3) / 1e12) if flop else None
     return {"kernel": name, "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": (achieved / peak) if achieved else None, "traffic": None, "per_launch_ms": per_launch_ms,
             "launches": launches, "work": unit_note,
-            "peak_note": "MI355X published vector peak for the kernel's arithmetic (FP32 157.3 for the mixed "
-                         "path, FP64 78.6); MFMA is not used"}
+            "peak_note": "MI355X published vector peak for the kernel's arithmetic (FP64 78.6 for the "
+                         "neighbor search and the FP64 loops, FP32 157.3 for the mixed loops); MFMA is not used"}
+
+
+def measured_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    profiles/traffic_<kernel>_<workload>.txt (FETCH_SIZE + WRITE_SIZE per dispatch, scaled
+    by the same counters' reading for a copy of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def calibrate_copy(nbytes):
+    """A plain device copy of known size (torch's copy kernel, nbytes read + nbytes written),
+    run under the PMC counters to scale FETCH_SIZE and WRITE_SIZE to bytes."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
 
 
 def cpu_baseline(side, steps):
@@ -152,7 +181,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one precision only (for profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (PMC byte-counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               f"(the single-rank path has no multi-GPU decomposition).", file=sys.stderr)
@@ -212,6 +246,8 @@ def main():
         rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
         out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
+    out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
+                                                  f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)

@@ -6,9 +6,11 @@
 //                   dimensions); overlapping leaves are collected in LDS, sorted
 //                   (leaf order = SFC order = particle order) and merged into
 //                   contiguous particle ranges.
-//   nsCandidates  : the ranges are streamed through LDS 64 particles at a time;
-//                   every lane tests every staged candidate against its own target
-//                   with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
+//   nsCandidates  : the ranges are streamed 64 particles at a time; each chunk is
+//                   filtered against the bounding boxes of the wave's four 16-target
+//                   quarters and compacted into per-quarter LDS lists, then lane
+//                   (quarter q, target t) tests quarter q's candidates against its
+//                   target with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
 //                   separately rounded operations (findneighbors.hpp:33-60,115-140),
 //                   so the neighbor sets are bit-identical to the CPU path; hits are
 //                   appended in ascending j, which is also the CPU's depth-first
@@ -32,7 +34,7 @@ struct NsFlags
     uint32_t           overflow;  // traversal capacity exceeded
     uint32_t           badGroup;  // a group with more than 64 particles
     uint32_t           pad;
-    unsigned long long tests;     // candidate pair tests (targets x candidates) of this pass
+    unsigned long long tests;     // exact (target, candidate) distance tests of this pass
 };
 
 __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
@@ -216,15 +218,6 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
         rcount += int(popc64(mS));
     }
     if (lane == 0) rangeCount[g] = uint32_t(rcount);
-    // candidate tests the candidate kernel will make for this group
-    unsigned long long cand = 0;
-    for (int r = int(lane); r < rcount; r += 64)
-    {
-        uint2 rg = ranges[size_t(g) * NS_LCAP + r];
-        cand += rg.y - rg.x;
-    }
-    cand = waveSum(cand);
-    if (lane == 0) atomicAdd(&flags->tests, cand * (unsigned long long)(e - s));
 }
 
 // updateH (sph/kernels.hpp:27-33)
@@ -254,7 +247,10 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                                                    NsFlags* __restrict__ flags)
 {
 #pragma clang fp contract(off)
-    __shared__ double sx[64], sy[64], sz[64];
+    // per quarter: the staged candidates that survive the quarter's box filter
+    __shared__ double   cx[4][64], cy[4][64], cz[4][64];
+    __shared__ uint32_t cj[4][64];
+    __shared__ double   qbox[4][7]; // centre xyz, half-width xyz, periodic flag
 
     const uint32_t g    = groupList ? groupList[blockIdx.x] : blockIdx.x;
     const unsigned lane = threadIdx.x;
@@ -279,15 +275,53 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     const double py     = box.pbc[1] ? box.len[1] : 0.0;
     const double pz     = box.pbc[2] ? box.len[2] : 0.0;
 
+    // quarter boxes: the bounding box of the 16 targets' 2h spheres, widened by a
+    // relative 1e-12 so that rounding never drops a candidate; a candidate that is
+    // a neighbor of any target of the quarter lies inside it (its image nearest the
+    // box centre, when the quarter uses periodic images)
+    const unsigned q = lane >> 4, t = lane & 15u;
+    {
+        double pp[3] = {xi, yi, zi};
+        double r     = 2.0 * hi;
+        double lo[3], hh[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            double a = act ? pp[d] - r : INFINITY, b = act ? pp[d] + r : -INFINITY;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1)
+            {
+                a = fmin(a, __shfl_xor(a, o, 64));
+                b = fmax(b, __shfl_xor(b, o, 64));
+            }
+            lo[d] = a, hh[d] = b;
+        }
+        uint64_t pbcMask = ballot(usePbc);
+        if (t == 0)
+        {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                double c = 0.5 * (lo[d] + hh[d]), w = 0.5 * (hh[d] - lo[d]);
+                qbox[q][d]     = c;
+                qbox[q][3 + d] = w + 1e-12 * (fabs(c) + w + box.len[d]);
+            }
+            qbox[q][6] = ((pbcMask >> (16u * q)) & 0xFFFFu) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+
     // non-SPH: reference CPU layout nbrs[(i - first) * ngmax + k]
     const size_t lbase = size_t(s + lane - G.first) * ngmax;
     // SPH: this lane's sub-group
-    const unsigned q      = lane >> 4, t = lane & 15u;
     const uint32_t sg     = 4u * g + q;
     uint16_t*      mySlot = Sph ? out.slots + size_t(sg) * ngmax * 16 + t : nullptr;
     uint32_t*      myUni  = Sph ? out.unions + size_t(sg) * out.capU : nullptr;
     uint32_t       usz    = 0;
     uint32_t       count  = 0;
+    const uint64_t lt     = lanemaskLt();
+    const uint32_t qAct   = uint32_t(popc64((ballot(act) >> (16u * q)) & 0xFFFFu));
+    unsigned long long tests = 0;
 
     const uint32_t R = rangeCount[g];
     for (uint32_t r = 0; r < R; ++r)
@@ -295,19 +329,44 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
         uint2 rg = ranges[size_t(g) * NS_LCAP + r];
         for (uint32_t base = rg.x; base < rg.y; base += 64)
         {
-            uint32_t m = min(64u, rg.y - base);
-            if (lane < m)
+            // stage: lane = candidate; filter against the 4 quarter boxes; compact
+            uint32_t m  = min(64u, rg.y - base);
+            bool     in = lane < m;
+            uint32_t j  = base + lane;
+            double   ax = 0, ay = 0, az = 0;
+            if (in) ax = x[j], ay = y[j], az = z[j];
+            uint32_t num[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
             {
-                sx[lane] = x[base + lane];
-                sy[lane] = y[base + lane];
-                sz[lane] = z[base + lane];
+                double dx = ax - qbox[qq][0], dy = ay - qbox[qq][1], dz = az - qbox[qq][2];
+                if (qbox[qq][6] != 0.0)
+                {
+                    dx -= px * rint(dx * box.ilen[0]);
+                    dy -= py * rint(dy * box.ilen[1]);
+                    dz -= pz * rint(dz * box.ilen[2]);
+                }
+                bool     pass = in && fabs(dx) <= qbox[qq][3] && fabs(dy) <= qbox[qq][4] && fabs(dz) <= qbox[qq][5];
+                uint64_t mk   = ballot(pass);
+                if (pass)
+                {
+                    uint32_t pos = uint32_t(popc64(mk & lt));
+                    cx[qq][pos] = ax, cy[qq][pos] = ay, cz[qq][pos] = az, cj[qq][pos] = j;
+                }
+                num[qq] = uint32_t(popc64(mk));
             }
             __syncthreads();
-            for (uint32_t k = 0; k < m; ++k)
+            const uint32_t myNum = num[q];
+            const uint32_t steps = max(max(num[0], num[1]), max(num[2], num[3]));
+            tests += (unsigned long long)myNum * qAct;
+            // test: lane (q, t) takes candidate k of quarter q's list against target t
+            for (uint32_t k = 0; k < steps; ++k)
             {
-                double dx = sx[k] - xi;
-                double dy = sy[k] - yi;
-                double dz = sz[k] - zi;
+                bool   valid = k < myNum;
+                uint32_t kk  = valid ? k : 0u;
+                double dx  = cx[q][kk] - xi;
+                double dy  = cy[q][kk] - yi;
+                double dz  = cz[q][kk] - zi;
                 if (usePbc)
                 {
                     dx -= px * rint(dx * box.ilen[0]);
@@ -315,25 +374,29 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                     dz -= pz * rint(dz * box.ilen[2]);
                 }
                 double   d2  = dx * dx + dy * dy + dz * dz;
-                uint32_t j   = base + k;
-                bool     hit = act && d2 < radiusSq && j != i;
+                uint32_t jj  = cj[q][kk];
+                bool     hit = act && valid && d2 < radiusSq && jj != i;
                 if constexpr (Sph)
                 {
                     bool     store = hit && count < ngmax;
                     uint32_t mq    = uint32_t(ballot(store) >> (16u * q)) & 0xFFFFu;
                     if (store) mySlot[size_t(count) * 16] = uint16_t(usz);
-                    if (mq && t == 0) myUni[usz] = j;
+                    if (mq && t == 0) myUni[usz] = jj;
                     usz += (mq != 0u);
                 }
                 else
                 {
-                    if (hit && nbrs && count < ngmax) nbrs[lbase + count] = j;
+                    if (hit && nbrs && count < ngmax) nbrs[lbase + count] = jj;
                 }
                 count += hit ? 1u : 0u;
             }
             __syncthreads();
         }
     }
+    // exact pair tests of this pass, per quarter (each quarter's count once)
+    tests = (t == 0) ? tests : 0ull;
+    tests = waveSum(tests);
+    if (lane == 0 && tests) atomicAdd(&flags->tests, tests);
 
     if constexpr (Sph)
     {

@@ -74,6 +74,50 @@ def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup):
     return step, ms, stages, kernels, pairs, tests
 
 
+def time_me_adjoint(step, reps=3):
+    """Momentum-energy adjoint w.r.t. positions after a step, with the neighbor lists
+    recomputed first (as a backward pass that does not keep the forward's lists)."""
+    import torch
+
+    g = [torch.randn(step.n, dtype=step.d.dtype, device=step.d.x.device) for _ in range(4)]
+    step.momentum_energy_vjp(*g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step.momentum_energy_vjp(*g)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def glass_relaxation(ctx, side=100, iterations=200):
+    """Differentiable density: gradient descent on sum (rho - mean rho)^2 over a side^3
+    periodic lattice with Gaussian jitter 0.3 spacing, fixed h, FP64 density adjoint."""
+    import numpy as np
+    import torch
+
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+    from sphexa_amd.workloads import jittered_lattice
+
+    n = side ** 3
+    f = jittered_lattice(side, 0.3, seed=4)[0]
+    dev = ctx.device
+    X = [torch.tensor(f[k], device=dev) for k in ("x", "y", "z")]
+    h = torch.full((n,), 0.5 * (300 / (4 * np.pi * n)) ** (1 / 3), dtype=torch.float64, device=dev)
+    m = torch.full((n,), 1.0 / n, dtype=torch.float64, device=dev)
+    box = hydro.make_box(0.0, 1.0, 1)
+    autodiff.relax_glass(ctx, box, *X, h, m, iterations=2, precision=FP64)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, _, hist = autodiff.relax_glass(ctx, box, *X, h, m, iterations=iterations, precision=FP64)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"particles": n, "iterations": iterations, "ms_per_iteration": dt * 1e3 / iterations,
+            "loss_first": hist[0], "loss_last": hist[-1],
+            "note": "per iteration: keys, sort, tree, groups, fixed-h search, density (forward) and the same "
+                    "with the density adjoint (backward, lists recomputed), FP64"}
+
+
 def roofline(kernels, pairs, n, steps, precision, tests):
     """Achieved flop rate of the dominant kernel from live HIP-event timings."""
     name, (launches, total) = max(kernels.items(), key=lambda kv: kv[1][1])
@@ -101,13 +145,13 @@ def roofline(kernels, pairs, n, steps, precision, tests):
 
 def measured_traffic(kernel, workload_tag):
     """HBM bytes per launch of `kernel` from the committed PMC summary
-    profiles/traffic_<kernel>_<workload>.txt (FETCH_SIZE + WRITE_SIZE per dispatch, scaled
-    by the same counters' reading for a copy of known size); None if absent."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.txt")
+    profiles/traffic_<workload>.txt (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by
+    the same counter's reading for a copy of known size); None if absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_tag}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("bytes_per_launch:"):
+        if line.startswith(f"bytes_per_launch {kernel}:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -181,6 +225,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one precision only (for profiling runs)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timings")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -211,6 +256,9 @@ def main():
                                                                 args.warmup)
         results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs,
                              rho=step.d.rho.double().clone(), tests=tests)
+        if not args.no_adjoint:
+            results[prec]["me_adjoint_ms"] = time_me_adjoint(step)
+            log(f"precision {prec}: momentum-energy adjoint {results[prec]['me_adjoint_ms']:.2f} ms")
         del step
         torch.cuda.empty_cache()
 
@@ -245,6 +293,16 @@ def main():
         out["mixed_over_fp64"] = F["ms"] / R["ms"]
         rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
         out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())
+    if "me_adjoint_ms" in R:
+        out["momentum_energy_adjoint"] = {
+            "ms": R["me_adjoint_ms"], "wrt": "x, y, z",
+            "note": "neighbor lists recomputed (search with the final h), records packed, adjoint kernel; "
+                    "same precision as value"}
+        if FP64 in results and "me_adjoint_ms" in results[FP64] and main_prec != FP64:
+            out["momentum_energy_adjoint"]["fp64_ms"] = results[FP64]["me_adjoint_ms"]
+    if not args.no_adjoint:
+        log("glass relaxation (density adjoint), 100^3")
+        out["density_adjoint_relaxation"] = glass_relaxation(ctx)
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")

@@ -60,6 +60,17 @@ struct GroupsDev
     const uint32_t* end   = nullptr;
 };
 
+// Blocks are dealt round-robin to the 8 XCDs of the device (block b runs on XCD
+// b mod 8).  This bijection of [0, numBlocks) gives each XCD one contiguous range of
+// work items, so that spatially adjacent items share that XCD's L2.
+__device__ inline uint32_t xcdContiguous(uint32_t b, uint32_t numBlocks)
+{
+    uint32_t perXcd = (numBlocks + 7u) / 8u;
+    uint32_t xcd = b & 7u, idx = b >> 3;
+    uint32_t full = numBlocks - 8u * (perXcd - 1u); // XCDs that get perXcd blocks
+    return xcd < full ? xcd * perXcd + idx : full * perXcd + (xcd - full) * (perXcd - 1u) + idx;
+}
+
 __host__ __device__ inline void groupRange(const GroupsDev& G, uint32_t g, uint32_t& s, uint32_t& e)
 {
     if (G.start)

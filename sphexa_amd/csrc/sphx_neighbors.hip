@@ -66,7 +66,8 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
     __shared__ int32_t front[2][NS_FCAP];
     __shared__ int32_t leafList[NS_LCAP];
 
-    const uint32_t g    = groupList ? groupList[blockIdx.x] : blockIdx.x;
+    const uint32_t blk  = xcdContiguous(blockIdx.x, gridDim.x);
+    const uint32_t g    = groupList ? groupList[blk] : blk;
     const unsigned lane = threadIdx.x;
     uint32_t       s, e;
     groupRange(G, g, s, e);
@@ -252,7 +253,8 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     __shared__ uint32_t cj[4][64];
     __shared__ double   qbox[4][7]; // centre xyz, half-width xyz, periodic flag
 
-    const uint32_t g    = groupList ? groupList[blockIdx.x] : blockIdx.x;
+    const uint32_t blk  = xcdContiguous(blockIdx.x, gridDim.x);
+    const uint32_t g    = groupList ? groupList[blk] : blk;
     const unsigned lane = threadIdx.x;
     uint32_t       s, e;
     groupRange(G, g, s, e);

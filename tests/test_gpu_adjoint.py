@@ -138,3 +138,108 @@ def test_glass_relaxation_reduces_density_variance(ctx):
     _, _, _, hist = autodiff.relax_glass(ctx, hb, *X, h, m, iterations=20, precision=FP64)
     print("loss", hist[0], hist[-1], flush=True)
     assert hist[-1] < 0.5 * hist[0]
+
+
+def _lattice20(ctx, seed=87):
+    """The differentiable-density case at 20^3: periodic unit lattice, Gaussian jitter
+    sigma = 0.3 spacing, m = 1/N, fixed h = 0.5 (300 / (4 pi N))^(1/3)."""
+    from sphexa_amd.workloads import jittered_lattice
+
+    side = 20
+    n = side ** 3
+    f = jittered_lattice(side, 0.3, seed=seed)[0]
+    h = np.full(n, 0.5 * (300 / (4 * np.pi * n)) ** (1 / 3))
+    m = np.full(n, 1.0 / n)
+    return f["x"], f["y"], f["z"], h, m
+
+
+def test_density_adjoint_matches_finite_differences(ctx):
+    """GPU adjoint of L = sum (rho - mean rho)^2 against central differences of the oracle's
+    density (fixed lists) on 64 random particles: <= 1e-6 of the gradient scale."""
+    import oracle
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+
+    print("case density_adjoint_fd", flush=True)
+    x, y, z, h, m = _lattice20(ctx)
+    hb = hydro.make_box(0.0, 1.0, 1)
+    op = autodiff.DensityOp(ctx, hb, FP64, ngmax=256)
+    xyz = [torch.tensor(a, device=ctx.device).requires_grad_(True) for a in (x, y, z)]
+    rho = op(*xyz, torch.tensor(h, device=ctx.device), torch.tensor(m, device=ctx.device))
+    grads = [g.cpu().numpy() for g in torch.autograd.grad(((rho - rho.mean()) ** 2).sum(), xyz)]
+    scale = max(np.abs(g).max() for g in grads)
+    # oracle density over the op's own lists, in the op's sorted order
+    d, order = op.pipe.d, op.pipe.order.long().cpu().numpy()
+    nb = hydro.export_neighbors(ctx, d).cpu().numpy().view(np.uint32)
+    nc = d.nc.cpu().numpy().view(np.uint32)
+    cpu = oracle.Cpu("oracle")
+    wh, _, K = cpu.kernel_tables()
+    box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
+    s = {k: np.ascontiguousarray(v[order]) for k, v in dict(x=x, y=y, z=z, h=h, m=m).items()}
+
+    def loss(f):
+        r = cpu.density(nb, nc, d.ngmax, f["x"], f["y"], f["z"], s["h"], s["m"], wh, K, box)
+        return float(np.sum((r - r.mean()) ** 2))
+
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    worst = 0.0
+    delta = 2e-9
+    for p in np.random.default_rng(88).choice(len(x), 64, replace=False):
+        for a, key in enumerate("xyz"):
+            plus, minus = dict(s), dict(s)
+            plus[key], minus[key] = s[key].copy(), s[key].copy()
+            plus[key][inv[p]] += delta
+            minus[key][inv[p]] -= delta
+            fd = (loss(plus) - loss(minus)) / (2 * delta)
+            worst = max(worst, abs(fd - grads[a][p]) / scale)
+    print("worst", worst, flush=True)
+    assert worst < 1e-6
+
+
+def test_density_adjoint_vs_bruteforce_autograd(ctx):
+    """GPU adjoint against FP64 torch autograd of an all-pairs density (no neighbor lists:
+    every pair, minimum image, the same table-linear W, which is 0 beyond 2h) at 20^3."""
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+
+    print("case density_adjoint_bruteforce", flush=True)
+    x, y, z, h, m = _lattice20(ctx, seed=89)
+    n = len(x)
+    hb = hydro.make_box(0.0, 1.0, 1)
+    op = autodiff.DensityOp(ctx, hb, FP64, ngmax=256)
+    dev = ctx.device
+    xyz = [torch.tensor(a, device=dev).requires_grad_(True) for a in (x, y, z)]
+    ht = torch.tensor(h, device=dev).requires_grad_(True)
+    mt = torch.tensor(m, device=dev)
+    w = torch.from_numpy(np.random.default_rng(90).normal(size=n)).to(dev)
+    rho = op(*xyz, ht, mt)
+    got = torch.autograd.grad((rho * w).sum(), xyz + [ht])
+    # brute force, rows in chunks
+    wh = torch.from_numpy(hydro.kernel_tables(6.0)[0]).to(dev)
+    K = op.K
+    bx = [torch.tensor(a, device=dev).requires_grad_(True) for a in (x, y, z)]
+    bh = torch.tensor(h, device=dev).requires_grad_(True)
+    parts = []
+    for r0 in range(0, n, 1000):
+        rows = slice(r0, min(n, r0 + 1000))
+        d2 = 0
+        for c in bx:
+            dd = c[rows, None] - c[None, :]
+            dd = dd - torch.round(dd)  # minimum image in the unit box
+            d2 = d2 + dd * dd
+        idx = torch.arange(r0, min(n, r0 + 1000), device=dev)
+        self_mask = torch.zeros_like(d2, dtype=torch.bool)
+        self_mask[torch.arange(len(idx), device=dev), idx] = True
+        d2 = torch.where(self_mask, torch.ones_like(d2), d2)  # keep sqrt differentiable on the diagonal
+        v = torch.sqrt(d2) / bh[rows, None]
+        Wv = R.table_w(wh, v.reshape(-1)).reshape(v.shape)
+        Wv = torch.where(self_mask, torch.zeros_like(Wv), Wv)
+        rho0 = mt[rows] + (Wv * mt[None, :]).sum(1)
+        parts.append(K / bh[rows] ** 3 * rho0)
+    rb = torch.cat(parts)
+    assert float((rb - rho.detach()).abs().max() / rb.abs().max()) < 1e-13
+    ref = torch.autograd.grad((rb * w).sum(), bx + [bh])
+    errs = [_err(a, b.cpu().double()) for a, b in zip(got, ref)]
+    print("errors", errs, flush=True)
+    assert max(errs) < 1e-11

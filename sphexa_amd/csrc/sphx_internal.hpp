@@ -32,7 +32,7 @@ enum Slot : int
     S_CS_LAYOUT,     // cornerstone: leaf layout
     S_NS_RANGES,     // neighbor search: candidate particle ranges per group
     S_NS_RCOUNT,     // neighbor search: range count per group
-    S_NS_LISTS,      // neighbor slot lists [sub-group][k][16]
+    S_NS_LISTS,      // neighbor bitmasks [sub-group][word][16]
     S_NS_UNION,      // neighbor unions per sub-group
     S_NS_USIZE,      // union sizes
     S_NS_STATE,      // per-particle h-iteration counters / active flags
@@ -98,15 +98,24 @@ __host__ __device__ inline size_t listBase(const ListLayout& lay, const GroupsDe
     return lay.cpu ? size_t(s + lane - G.first) * ngmax : size_t(g) * ngmax * 64 + lane;
 }
 
-// SPH neighbor lists: per 16-target sub-group sg, the sorted union of its targets'
-// stored neighbors and per target the u16 positions of its neighbors in the union
+// SPH neighbor lists: per 16-target sub-group sg (targets 16 q .. 16 q + 15 of group
+// g, sg = 4 g + q), the ascending union of the targets' stored neighbors, and per
+// target a bitmask over that union: bit b of masks[sg][w][t] set = union entry
+// 32 w + b is a stored neighbor of target t.  Set bits ascend with j, so a target's
+// neighbors read in bit order come in ascending j, the reference's list order.
 struct SphLists
 {
-    uint16_t* slots  = nullptr; // [sg][k][16]
+    uint32_t* masks  = nullptr; // [sg][words][16]
     uint32_t* unions = nullptr; // [sg][capU]
     uint32_t* usize  = nullptr; // [sg]
-    uint32_t  capU   = 0;       // 16 * ngmax
+    uint32_t  capU   = 0;       // 16 * ngmax, a multiple of 32
+    uint32_t  words  = 0;       // capU / 32
 };
+
+__host__ __device__ inline const uint32_t* maskRow(const SphLists& L, uint32_t sg, uint32_t t)
+{
+    return L.masks + size_t(sg) * L.words * 16 + t;
+}
 
 // entries of a particle's list that the loops read: min(nc - 1, ngmax), 0 when nc == 0
 __host__ __device__ inline uint32_t listCount(uint32_t ncSelf, uint32_t ngmax)

@@ -233,10 +233,11 @@ __device__ inline double updateH(unsigned ng0, unsigned nc, double h)
 //
 // SPH list format, per sub-group of 16 consecutive targets (quarter q of the wave,
 // sg = 4 g + q): the sorted union of the stored neighbors of its 16 targets
-// (unions[sg][0..usize)) and per target the ascending u16 positions of its
-// neighbors in that union (slots[sg][k][t]).  Candidates arrive in ascending j, so
-// a union entry is appended the first time any target of the quarter hits it, and
-// slot order equals j order.
+// (unions[sg][0..usize)) and per target a bitmask over that union
+// (masks[sg][w][t]).  Each quarter's candidates arrive in ascending j, so a union
+// entry is appended the first time any target of the quarter hits it; the lanes of
+// the quarter build their mask words in registers and store one coalesced 64-byte
+// row per 32 union entries.
 template<class Th, bool Sph>
 __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                    const double* __restrict__ x, const double* __restrict__ y,
@@ -317,9 +318,10 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     const size_t lbase = size_t(s + lane - G.first) * ngmax;
     // SPH: this lane's sub-group
     const uint32_t sg     = 4u * g + q;
-    uint16_t*      mySlot = Sph ? out.slots + size_t(sg) * ngmax * 16 + t : nullptr;
+    uint32_t*      myMask = Sph ? out.masks + size_t(sg) * out.words * 16 + t : nullptr;
     uint32_t*      myUni  = Sph ? out.unions + size_t(sg) * out.capU : nullptr;
     uint32_t       usz    = 0;
+    uint32_t       word   = 0; // mask bits of union entries 32 * (usz / 32) ...
     uint32_t       count  = 0;
     const uint64_t lt     = lanemaskLt();
     const uint32_t qAct   = uint32_t(popc64((ballot(act) >> (16u * q)) & 0xFFFFu));
@@ -382,9 +384,17 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                 {
                     bool     store = hit && count < ngmax;
                     uint32_t mq    = uint32_t(ballot(store) >> (16u * q)) & 0xFFFFu;
-                    if (store) mySlot[size_t(count) * 16] = uint16_t(usz);
-                    if (mq && t == 0) myUni[usz] = jj;
-                    usz += (mq != 0u);
+                    if (mq)
+                    {
+                        if (store) word |= 1u << (usz & 31u);
+                        if (t == 0) myUni[usz] = jj;
+                        ++usz;
+                        if ((usz & 31u) == 0)
+                        {
+                            myMask[size_t(usz / 32u - 1u) * 16] = word;
+                            word = 0;
+                        }
+                    }
                 }
                 else
                 {
@@ -403,6 +413,7 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     if constexpr (Sph)
     {
         if (t == 0) out.usize[sg] = usz;
+        if (usz & 31u) myMask[size_t(usz / 32u) * 16] = word;
         bool redo = false;
         if (act)
         {
@@ -462,12 +473,21 @@ __global__ __launch_bounds__(256) void nsExport(GroupsDev G, SphLists L, uint32_
     groupRange(G, g, s, e);
     uint32_t i = s + lane;
     if (i >= e || lane >= 64) return;
-    uint32_t        cnt  = listCount(nc[i], ngmax);
     uint32_t        sg   = 4u * g + (lane >> 4);
-    const uint16_t* sl   = L.slots + size_t(sg) * ngmax * 16 + (lane & 15u);
+    const uint32_t* M    = maskRow(L, sg, lane & 15u);
     const uint32_t* uni  = L.unions + size_t(sg) * L.capU;
-    for (uint32_t k = 0; k < cnt; ++k)
-        out[size_t(i - G.first) * ngmax + k] = uni[sl[size_t(k) * 16]];
+    const uint32_t  nw   = (L.usize[sg] + 31u) / 32u;
+    uint32_t        k    = 0;
+    for (uint32_t w = 0; w < nw; ++w)
+    {
+        uint32_t m = M[size_t(w) * 16];
+        while (m)
+        {
+            uint32_t b = uint32_t(__builtin_ctz(m));
+            m &= m - 1u;
+            out[size_t(i - G.first) * ngmax + k++] = uni[32u * w + b];
+        }
+    }
 }
 
 
@@ -713,11 +733,12 @@ int sphx_find_neighbors_sph(sphx_ctx* ctx, const sphx_groups* groups, sphx_parti
     ctx->nsValid = false;
     size_t   numSub = size_t(G.numGroups) * 4;
     SphLists L;
-    L.capU   = 16u * d->ngmax;
-    L.slots  = scratchT<uint16_t>(ctx, S_NS_LISTS, numSub * d->ngmax * 16);
+    L.capU   = 32u * ((16u * d->ngmax + 31u) / 32u);
+    L.words  = L.capU / 32u;
+    L.masks  = scratchT<uint32_t>(ctx, S_NS_LISTS, numSub * L.words * 16);
     L.unions = scratchT<uint32_t>(ctx, S_NS_UNION, numSub * L.capU);
     L.usize  = scratchT<uint32_t>(ctx, S_NS_USIZE, numSub);
-    if (!L.slots || !L.unions || !L.usize) return SPHX_ERR_NOMEM;
+    if (!L.masks || !L.unions || !L.usize) return SPHX_ERR_NOMEM;
     int st;
     if (d->precision == SPHX_FP64)
         st = runSearch<double, true>(ctx, G, d->x, d->y, d->z, static_cast<double*>(d->h), box, tree, d->ngmax,

@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsphx.so")
+# the in-tree build; SPHX_LIBRARY names another build of the same library (e.g. a
+# variant compiled with different kernel options) to load instead
+LIB_PATH = os.environ.get("SPHX_LIBRARY") or os.path.join(HERE, "libsphx.so")
 
 SPHX_OK, SPHX_ERR_HIP, SPHX_ERR_ARG, SPHX_ERR_STACK, SPHX_ERR_NGMAX, SPHX_ERR_STATE, SPHX_ERR_NOMEM, \
     SPHX_ERR_NODEV = range(8)

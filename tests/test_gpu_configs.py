@@ -1,0 +1,245 @@
+"""The workload configurations at reduced size against the oracle (bit-exact lists, FP64
+tolerances), and at full size through size-independent checks on sampled particles:
+
+* neighbor counts against a k-d tree count (scipy cKDTree, periodic where the box is)
+  with the same smoothing lengths, and symmetric membership where h is uniform;
+* density, IAD and momentum-energy of sampled particles recomputed by the oracle's
+  one-particle loops on the GPU's own neighbor lists and fields.
+
+Sizes: uniform random 1M with ~800 neighbors (ngmax 1024, fixed h), Evrard-like 4M
+(open box, 1/r density, mixed and FP64), turbulence 256^3 (mixed)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from sphexa_amd import workloads
+from tests.helpers import oracle_step, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CPU = oracle.Cpu("oracle")
+ME_F = ("vx", "vy", "vz", "h", "m", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from sphexa_amd import hydro
+
+    c = hydro.Context(0)
+    yield c
+    c.close()
+
+
+def _boxes(spec):
+    """(oracle.Box, sphx Box) from a workload box tuple (lo, hi, periodic)."""
+    from sphexa_amd import hydro
+
+    lo, hi, per = spec
+    bc = oracle.BC_PERIODIC if per else oracle.BC_OPEN
+    lo3 = [lo] * 3 if np.isscalar(lo) else list(lo)
+    hi3 = [hi] * 3 if np.isscalar(hi) else list(hi)
+    return oracle.Box.make(lo3, hi3, [bc] * 3), hydro.make_box(lo3, hi3, bc)
+
+
+def _gpu_step(ctx, f, hbox, precision, ngmax=150, ng0=100, exact=False):
+    from sphexa_amd import hydro
+
+    n = len(f["x"])
+    ctx.set_fp64_exact(exact)
+    step = hydro.HydroStep(ctx, n, hbox, precision, ng0=ng0, ngmax=ngmax)
+    dt = step.d.dtype
+    d = step.compute_forces({k: torch.tensor(v, dtype=torch.float64 if k in ("x", "y", "z") else dt,
+                                             device=ctx.device) for k, v in f.items()})
+    torch.cuda.synchronize()
+    ctx.set_fp64_exact(False)
+    return step, d
+
+
+def _sampled_rows(ctx, d, idx):
+    """Neighbor rows of particles idx (reference CPU layout) from the context's lists."""
+    from sphexa_amd import hydro
+
+    full = hydro.export_neighbors(ctx, d).view(d.n, d.ngmax)
+    rows = full[torch.as_tensor(idx, device=full.device, dtype=torch.long)].cpu().numpy().view(np.uint32)
+    del full
+    torch.cuda.empty_cache()
+    return rows
+
+
+def _local(ids, fields):
+    """Fields restricted to the particle set ids (target first), as contiguous float64."""
+    return {k: np.ascontiguousarray(v[ids], dtype=np.float64) for k, v in fields.items()}
+
+
+def _check_sampled_loops(ctx, d, obox, idx, tol_rho, tol_iad, tol_acc, wh, K):
+    """Oracle one-particle loops on the GPU's lists and inputs for particles idx."""
+    host = {k: getattr(d, k).cpu().double().numpy() for k in ("x", "y", "z") + ME_F + ("ax", "ay", "az", "du")}
+    nc = d.nc.cpu().numpy().view(np.uint32)
+    rows = _sampled_rows(ctx, d, idx)
+    worst = {"rho": 0.0, "iad": 0.0, "acc": 0.0, "du": 0.0}
+    iad_scale = max(np.abs(host[c]).max() for c in ("c11", "c22", "c33"))
+    acc_scale = max(np.abs(host[c]).max() for c in ("ax", "ay", "az"))
+    du_scale = max(np.abs(host["du"]).max(), 1e-300)  # du = 0 everywhere when v = 0
+    for r, i in enumerate(idx):
+        cnt = int(min(max(int(nc[i]) - 1, 0), d.ngmax))
+        nb = rows[r, :cnt].astype(np.int64)
+        ids = np.concatenate([[i], nb])
+        loc = _local(ids, host)
+        lnb = np.arange(1, cnt + 1, dtype=np.uint32)
+        xm = CPU.xmass_one(0, lnb, loc["x"], loc["y"], loc["z"], loc["h"], loc["m"], wh, K, obox)
+        rho = loc["m"][0] / xm
+        worst["rho"] = max(worst["rho"], abs(rho - host["rho"][i]) / host["rho"][i])
+        out6 = np.zeros(6)
+        CPU._iad_one(0, oracle._p(lnb), cnt, *[oracle._p(loc[k]) for k in ("x", "y", "z", "h", "m", "rho")],
+                     oracle._p(wh), K, oracle.C.byref(obox), oracle._p(out6))
+        got = np.array([host[c][i] for c in ("c11", "c12", "c13", "c22", "c23", "c33")])
+        worst["iad"] = max(worst["iad"], np.abs(got - out6).max() / iad_scale)
+        out5 = np.zeros(5)
+        keys = ("x", "y", "z", "vx", "vy", "vz", "h", "m", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33")
+        CPU._momentum_energy_one(0, oracle._p(lnb), cnt, *[oracle._p(loc[k]) for k in keys], oracle._p(wh), K,
+                                 oracle.C.byref(obox), oracle._p(out5))
+        worst["acc"] = max(worst["acc"], max(abs(host[c][i] - out5[k]) for k, c in enumerate(("ax", "ay", "az")))
+                           / acc_scale)
+        err_du = abs(host["du"][i] - out5[3])
+        worst["du"] = max(worst["du"], err_du / du_scale if err_du > 0 else 0.0)
+    print("sampled errors", worst, flush=True)
+    assert worst["rho"] < tol_rho
+    assert worst["iad"] < tol_iad
+    assert worst["acc"] < tol_acc and worst["du"] < tol_acc
+
+
+def _kdtree_counts(f, h, idx, periodic, box_len=1.0):
+    """Neighbors j != i with |x_i - x_j| < 2 h_i, counted with a k-d tree."""
+    from scipy.spatial import cKDTree
+
+    pts = np.stack([f["x"], f["y"], f["z"]], axis=1)
+    tree = cKDTree(pts, boxsize=box_len if periodic else None)
+    out = np.empty(len(idx), np.int64)
+    for r, i in enumerate(idx):
+        out[r] = len(tree.query_ball_point(pts[i], 2.0 * h[i] * (1 - 1e-15))) - 1
+    return out
+
+
+# ----------------------------------------------------------------------------- uniform random, ~800 neighbors
+def test_uniform_800_neighbors_plain_search_bit_exact(ctx):
+    """Reduced size (100k): plain findNeighbors (fixed h, ~800 neighbors, ngmax 1024) lists
+    and counts bit-exact against the oracle."""
+    from sphexa_amd import hydro
+
+    print("case uniform800 reduced", flush=True)
+    f, spec = workloads.uniform_random(100_000, seed=101)
+    obox, hbox = _boxes(spec)
+    n = len(f["x"])
+    keys = CPU.sfc_keys(f["x"], f["y"], f["z"], obox)
+    keys, order = CPU.sort_by_key(keys, np.arange(n, dtype=np.uint32))
+    s = {k: np.ascontiguousarray(v[order]) for k, v in f.items()}
+    rt = CPU.tree(keys, obox)
+    ref_nb, ref_nc = CPU.find_neighbors(s["x"], s["y"], s["z"], s["h"], obox, rt, 1024)
+    assert ref_nc.max() <= 1024 and ref_nc.mean() > 700
+    t = hydro.tree_from_keys(ctx, torch.from_numpy(keys.view(np.int64)).cuda(), hbox)
+    dv = lambda a: torch.from_numpy(a).cuda()
+    nb, nc = hydro.find_neighbors(ctx, dv(s["x"]), dv(s["y"]), dv(s["z"]), dv(s["h"]), hbox, t, 1024)
+    nc = nc.cpu().numpy().view(np.uint32)
+    assert np.array_equal(nc, ref_nc)
+    nb = nb.cpu().numpy().view(np.uint32).reshape(n, 1024)
+    for i in range(0, n, 97):
+        assert np.array_equal(nb[i, :nc[i]], ref_nb[i, :nc[i]]), i
+
+
+def test_uniform_1M_800_neighbors_full_size(ctx):
+    """Full size (1M, ~800 neighbors, ngmax 1024, FP64 step): counts against a k-d tree and
+    symmetric membership on sampled particles; density, IAD and momentum-energy of sampled
+    particles against the oracle's loops on the GPU lists."""
+    from sphexa_amd._lib import FP64
+
+    print("case uniform800 full", flush=True)
+    f, spec = workloads.uniform_random(1_000_000, seed=102)
+    obox, hbox = _boxes(spec)
+    step, d = _gpu_step(ctx, f, hbox, FP64, ngmax=1024, ng0=100)
+    # the h iteration must not have fired: every count lies in [ng0/4, ngmax + 1]
+    nc = d.nc.cpu().numpy().view(np.uint32).astype(np.int64)
+    assert nc.min() - 1 >= 25 and nc.max() - 1 <= 1024
+    h = d.h.cpu().numpy()
+    sf = {k: getattr(d, k).cpu().numpy() for k in ("x", "y", "z")}
+    idx = np.random.default_rng(103).choice(d.n, 1500, replace=False)
+    assert np.array_equal(nc[idx] - 1, _kdtree_counts(sf, h, idx, True))
+    # uniform h: j in N(i) <=> i in N(j)
+    rows = _sampled_rows(ctx, d, idx[:200])
+    pairs = [(i, int(j)) for r, i in enumerate(idx[:200]) for j in rows[r, :nc[i] - 1][::37]]
+    back = _sampled_rows(ctx, d, [j for _, j in pairs])
+    for (i, j), rj in zip(pairs, back):
+        assert i in rj[:nc[j] - 1], (i, j)
+    wh, _, K = CPU.kernel_tables()
+    _check_sampled_loops(ctx, d, obox, idx[:300], 1e-13, 1e-10, 1e-10, wh, K)
+
+
+# ----------------------------------------------------------------------------- Evrard-like cloud
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_evrard_reduced_vs_oracle(ctx, mode):
+    """Reduced size (40k): open box, 1/r density, h iteration from ~100 neighbors,
+    ngmax 512: the whole step against the oracle."""
+    from sphexa_amd._lib import FP64, MIXED
+
+    print("case evrard reduced", mode, flush=True)
+    f, spec = workloads.evrard(40_000, seed=104)
+    obox, hbox = _boxes(spec)
+    ref = oracle_step(f, obox, ngmax=512)
+    step, d = _gpu_step(ctx, f, hbox, FP64 if mode == "fp64" else MIXED, ngmax=512)
+    got = {k: getattr(d, k).double().cpu().numpy() for k in ("rho", "c11", "c22", "ax", "du")}
+    if mode == "fp64":
+        assert rel_err(d.h.cpu().numpy(), ref["h"]) < 1e-14
+        same = d.h.cpu().numpy() == ref["h"]
+        assert same.mean() > 0.75
+        assert np.array_equal(d.nc.cpu().numpy().view(np.uint32)[same], ref["nc"][same])
+        assert rel_err(got["rho"], ref["rho"]) < 1e-12
+    else:
+        assert rel_err(got["rho"], ref["rho"]) < 1e-5
+    spread = np.percentile(ref["nc"], 99) / max(1, np.percentile(ref["nc"], 1))
+    print("nc spread p99/p1", spread, flush=True)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_evrard_4M_full_size(ctx, mode):
+    """Full size (4M, open box, ngmax 512): counts against a k-d tree with the final h;
+    sampled loops against the oracle on the GPU's lists and inputs."""
+    from sphexa_amd._lib import FP64, MIXED
+
+    print("case evrard full", mode, flush=True)
+    f, spec = workloads.evrard(4_194_304, seed=105)
+    obox, hbox = _boxes(spec)
+    step, d = _gpu_step(ctx, f, hbox, FP64 if mode == "fp64" else MIXED, ngmax=512)
+    nc = d.nc.cpu().numpy().view(np.uint32).astype(np.int64)
+    assert nc.max() - 1 <= 512
+    h = d.h.cpu().double().numpy()
+    sf = {k: getattr(d, k).cpu().numpy() for k in ("x", "y", "z")}
+    idx = np.random.default_rng(106).choice(d.n, 1000, replace=False)
+    assert np.array_equal(nc[idx] - 1, _kdtree_counts(sf, h, idx, False))
+    wh, _, K = CPU.kernel_tables()
+    if mode == "fp64":
+        _check_sampled_loops(ctx, d, obox, idx[:300], 1e-13, 1e-10, 1e-10, wh, K)
+    else:
+        _check_sampled_loops(ctx, d, obox, idx[:300], 1e-5, 1e-4, 1e-3, wh, K)
+
+
+# ----------------------------------------------------------------------------- turbulence 256^3
+def test_turbulence_256_full_size_mixed(ctx):
+    """The benchmark workload (16.8M, mixed): counts against a k-d tree and sampled loops
+    against the oracle on the GPU's lists and inputs."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import MIXED
+
+    print("case turbulence256 full", flush=True)
+    inp, _ = workloads.turbulence(256, device=ctx.device)
+    obox, hbox = _boxes((0.0, 1.0, True))
+    step = hydro.HydroStep(ctx, 256 ** 3, hbox, MIXED)
+    d = step.compute_forces({k: (v if k in ("x", "y", "z") else v.float()) for k, v in inp.items()})
+    torch.cuda.synchronize()
+    del inp
+    nc = d.nc.cpu().numpy().view(np.uint32).astype(np.int64)
+    h = d.h.cpu().double().numpy()
+    sf = {k: getattr(d, k).cpu().numpy() for k in ("x", "y", "z")}
+    idx = np.random.default_rng(107).choice(d.n, 600, replace=False)
+    assert np.array_equal(nc[idx] - 1, _kdtree_counts(sf, h, idx, True))
+    wh, _, K = CPU.kernel_tables()
+    _check_sampled_loops(ctx, d, obox, idx[:200], 1e-5, 1e-4, 1e-3, wh, K)

@@ -10,7 +10,11 @@ Workload: subsonic turbulence box, 256^3 = 16,777,216 particles on a jittered
 periodic lattice (sigma = 0.1 spacing), velocities a Gaussian random field with
 P(k) ~ k^(-11/3) for 1 <= |k| <= 3 at rms Mach 0.3, u = 1, h for ~100 neighbors.
 `value` is the mixed-precision step's particles/s; the FP64 step is timed in the same
-run and reported beside it.
+run and reported beside it.  Also reported: the momentum-energy adjoint with respect to
+positions after a step (lists recomputed), the glass relaxation of a 100^3 jittered
+lattice with the density adjoint (200 iterations), the roofline of the dominant kernel
+(live HIP-event timing; HBM traffic from the committed PMC summary in profiles/), and
+the reference CPU path timed on the host cores on the same input (one step).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 256]
 Prints one JSON line on stdout; progress goes to stderr.
@@ -169,24 +173,23 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
 
 
-def cpu_baseline(side, steps):
-    """The reference CPU path (oracle/_ref, compiled from the reference sources) on a
-    bounded sample of the same workload, on the host cores; the oracle port if the
+def cpu_baseline(f, side, steps):
+    """The reference CPU path (oracle/_ref, compiled from the reference sources) on the
+    host cores, on the same input as the GPU (host copy of f); the oracle port if the
     reference build is absent."""
     import ctypes as C
 
     import numpy as np
 
     import oracle
-    from sphexa_amd.workloads import turbulence
 
     kind = "reference" if oracle.available("ref") else "port"
-    f, _ = turbulence(side)
     n = len(f["x"])
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
     p = lambda a: np.ascontiguousarray(a).ctypes.data_as(C.c_void_p)
     times = []
+    stages = None
     if kind == "reference":
         r = oracle.Cpu("ref")
         rho, ax, du = np.zeros(n), np.zeros(n), np.zeros(n)
@@ -198,6 +201,7 @@ def cpu_baseline(side, steps):
                           p(f["vz"]), p(f["u"]), C.byref(box), 64, 100, 150, 6.0, 5.0 / 3.0, 0.2, p(rho), p(ax),
                           p(du), p(nc), p(st))
             times.append(time.perf_counter() - t0)
+            stages = [round(float(v), 3) for v in st]
             log(f"cpu baseline step {s + 1}/{steps}: {times[-1]:.2f} s")
     else:
         from tests.helpers import oracle_step
@@ -208,10 +212,15 @@ def cpu_baseline(side, steps):
             times.append(time.perf_counter() - t0)
             log(f"cpu baseline (port) step {s + 1}/{steps}: {times[-1]:.2f} s")
     t = sorted(times)[len(times) // 2]
-    return {"value": n / t, "unit": "particles/s", "cores": threads, "kind": kind,
-            "sample": f"turbulence {side}^3 = {n} particles (same generator, FP64), median of {steps} full "
-                      f"steps of the reference CPU path (keys, sort, tree, findNeighborsSph, density, EOS, IAD, "
-                      f"momentum-energy), {threads} OpenMP threads"}
+    out = {"value": n / t, "unit": "particles/s", "cores": threads, "kind": kind,
+           "sample": f"turbulence {side}^3 = {n} particles, the GPU run's own input copied to the host (FP64), "
+                     f"median of {steps} full step(s) of the reference CPU path (keys, sort, tree, "
+                     f"findNeighborsSph, density, EOS, IAD, momentum-energy), {threads} OpenMP threads",
+           "seconds_per_step": t}
+    if stages is not None:
+        out["stages_s"] = dict(zip(("keys", "sort_gather", "octree", "link_centers", "neighbors", "density", "eos",
+                                    "iad", "momentum_energy"), stages))
+    return out
 
 
 def main():
@@ -220,8 +229,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--side", type=int, default=256, help="lattice side; N = side^3")
-    ap.add_argument("--cpu-side", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-side", type=int, default=0, help="CPU baseline lattice side (default: --side)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one precision only (for profiling runs)")
@@ -308,7 +317,11 @@ def main():
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")
     if not args.no_cpu:
         log("timing the CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_side, args.cpu_steps)
+        cs = args.cpu_side or args.side
+        src = inp64 if cs == args.side else turbulence(cs, device=ctx.device)[0]
+        host = {k: v.double().cpu().numpy() for k, v in src.items()}
+        del src
+        out["cpu_baseline"] = cpu_baseline(host, cs, args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -163,6 +163,14 @@ int sphx_timing_summary(sphx_ctx* ctx, int cap, const char** names, int* launche
  * in float.  Any output pointer may be NULL. */
 int sphx_kernel_tables(double sincIndex, double* wh64, float* wh32, double* K);
 
+/* ---- coordinate box ------------------------------------------------------ */
+/* cstone::makeGlobalBox (domain/include/cstone/sfc/box_mpi.hpp:68-107) on one rank:
+ * periodic and fixed dimensions keep the limits of *previous; open dimensions get
+ * the min and max of the n coordinates (unchanged when n == 0).  Synchronises the
+ * stream once (the box is returned on the host). */
+int sphx_make_global_box(sphx_ctx* ctx, const double* x, const double* y, const double* z, uint64_t n,
+                         const sphx_box* previous, sphx_box* out);
+
 /* ---- SFC keys ------------------------------------------------------------ */
 /* cstone::computeSfcKeys<true> (domain/include/cstone/sfc/sfc_gpu.h:26-31,
  * CPU semantics sfc/sfc.hpp:267-275): 64-bit Hilbert keys; entries already equal

@@ -73,6 +73,17 @@ auto rawPtr(V& v) -> decltype(v.data())
     return v.data();
 }
 
+// ---- cstone::makeGlobalBox (sfc/box_mpi.hpp:68-107) on one rank: a Box of the caller's
+// type with open dimensions fitted to the coordinates
+template<class Box>
+Box makeGlobalBox(const double* x, const double* y, const double* z, size_t n, const Box& previous)
+{
+    sphx_box p = toBox(previous), r{};
+    check(sphx_make_global_box(context(), x, y, z, n, &p, &r));
+    return Box(r.xmin, r.xmax, r.ymin, r.ymax, r.zmin, r.zmax, previous.boundaryX(), previous.boundaryY(),
+               previous.boundaryZ());
+}
+
 // ---- cstone::computeSfcKeys<true> (domain/include/cstone/sfc/sfc_gpu.h:26-31)
 template<class Box>
 void computeSfcKeys(const double* x, const double* y, const double* z, uint64_t* keys, size_t n, const Box& box)

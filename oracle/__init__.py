@@ -94,6 +94,7 @@ class Cpu:
             f("eos", None, [_U, _U, _V, _V, _D, _V, _V])
             f("iad", None, [_U, _U, _V, _V, _U] + [_V] * 7 + [_D, _V] + [_V] * 6)
             f("momentum_energy", _D, [_U, _U, _V, _V, _U] + [_V] * 18 + [_D, C.c_float, _V] + [_V] * 4)
+            f("make_global_box", None, [_V, _V, _V, _U64, _V, _V])
             f("compute_groups", _I, [_U, _U, _V, _V, _V, _V, _V, _I, _V, _V, C.c_float, _V, _I])
             f("iad_one", None, [_U, _V, _U] + [_V] * 7 + [_D, _V, _V])
             f("momentum_energy_one", None, [_U, _V, _U] + [_V] * 18 + [_D, _V, _V])
@@ -204,6 +205,11 @@ class Cpu:
                                   _p(b), cap)
         assert ng >= 0
         return b[:ng + 1]
+
+    def make_global_box(self, x, y, z, prev):
+        out = Box()
+        self._make_global_box(_p(x), _p(y), _p(z), len(x), C.byref(prev), C.byref(out))
+        return out
 
     # ---------------------------------------------------------------- neighbors
     def find_neighbors(self, x, y, z, h, box, t, ngmax, first=0, last=None, lists=True):

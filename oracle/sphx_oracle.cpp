@@ -1006,4 +1006,28 @@ int orc_compute_groups(uint32_t first, uint32_t last, const double* x, const dou
     return numGroups;
 }
 
+// makeGlobalBox (domain/include/cstone/sfc/box_mpi.hpp:33-107) on one rank: periodic and
+// fixed dimensions keep the previous limits, open ones take min / max of the coordinates
+void orc_make_global_box(const double* x, const double* y, const double* z, uint64_t n, const sphx_box* prev,
+                         sphx_box* out)
+{
+    *out = *prev;
+    if (n == 0) return;
+    const double* c[3] = {x, y, z};
+    int32_t bc[3] = {prev->bx, prev->by, prev->bz};
+    double* lo[3] = {&out->xmin, &out->ymin, &out->zmin};
+    double* hi[3] = {&out->xmax, &out->ymax, &out->zmax};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (bc[d] == SPHX_BC_PERIODIC || bc[d] == SPHX_BC_FIXED) continue;
+        double mn = INFINITY, mx = -INFINITY;
+        for (uint64_t i = 0; i < n; ++i)
+        {
+            mn = std::min(mn, c[d][i]);
+            mx = std::max(mx, c[d][i]);
+        }
+        *lo[d] = mn, *hi[d] = mx;
+    }
+}
+
 } // extern "C"

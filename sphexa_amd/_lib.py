@@ -93,6 +93,7 @@ SIGNATURES = {
                                        _V, _V]),
     "sphx_momentum_energy_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V,
                                                _V, _V, _V, _V, _V, _V]),
+    "sphx_make_global_box": (C.c_int, [_V, _V, _V, _V, _U64, C.POINTER(Box), C.POINTER(Box)]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),
                                       C.POINTER(C.c_int)]),

@@ -428,6 +428,55 @@ int gatherT(sphx_ctx* ctx, const uint32_t* order, const T* in, T* out, uint64_t 
     return SPHX_OK;
 }
 
+// coordinate extrema for makeGlobalBox: per block min and max of x, y, z (6 values)
+__global__ __launch_bounds__(256) void boxExtremaKernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, uint64_t n,
+                                                        double* __restrict__ partial)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    {
+        double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            lo[d] = v[d] < lo[d] ? v[d] : lo[d]; // std::min / std::max order (MinMax, box_mpi.hpp:34-52)
+            hi[d] = hi[d] < v[d] ? v[d] : hi[d];
+        }
+    }
+    __shared__ double red[6][4];
+    const unsigned    lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        double a = waveMin(lo[d]), b = waveMax(hi[d]);
+        if (lane == 0) red[d][w] = a, red[3 + d][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+    {
+        int    k = threadIdx.x;
+        double v = red[k][0];
+        for (int q = 1; q < 4; ++q)
+            v = k < 3 ? (red[k][q] < v ? red[k][q] : v) : (v < red[k][q] ? red[k][q] : v);
+        partial[size_t(blockIdx.x) * 6 + k] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void boxExtremaFinal(const double* __restrict__ partial, uint32_t blocks,
+                                                      double* __restrict__ out)
+{
+    const unsigned k = threadIdx.x;
+    if (k >= 6) return;
+    double v = k < 3 ? INFINITY : -INFINITY;
+    for (uint32_t b = 0; b < blocks; ++b)
+    {
+        double p = partial[size_t(b) * 6 + k];
+        v        = k < 3 ? (p < v ? p : v) : (v < p ? p : v);
+    }
+    out[k] = v;
+}
+
 } // namespace sphx
 
 using namespace sphx;
@@ -569,6 +618,37 @@ int sphx_gather_f32(sphx_ctx* ctx, const uint32_t* order, const float* in, float
 int sphx_gather_u32(sphx_ctx* ctx, const uint32_t* order, const uint32_t* in, uint32_t* out, uint64_t n)
 {
     return gatherT(ctx, order, in, out, n);
+}
+
+int sphx_make_global_box(sphx_ctx* ctx, const double* x, const double* y, const double* z, uint64_t n,
+                         const sphx_box* previous, sphx_box* out)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!previous || !out) return argError(ctx, "make_global_box: null box");
+    const int32_t bc[3] = {previous->bx, previous->by, previous->bz};
+    bool          keep[3];
+    for (int d = 0; d < 3; ++d)
+        keep[d] = bc[d] == SPHX_BC_PERIODIC || bc[d] == SPHX_BC_FIXED;
+    sphx_box r = *previous;
+    if (n > 0 && !(keep[0] && keep[1] && keep[2]))
+    {
+        if (!x || !y || !z) return argError(ctx, "make_global_box: null coordinates");
+        uint32_t blocks  = uint32_t(std::min<uint64_t>(gridFor(n, 256), 1024));
+        double*  partial = scratchT<double>(ctx, S_SMALL, size_t(blocks) * 6 + 6);
+        if (!partial) return SPHX_ERR_NOMEM;
+        boxExtremaKernel<<<blocks, 256, 0, ctx->stream>>>(x, y, z, n, partial);
+        SPHX_LAUNCH_CHECK(ctx, "boxExtremaKernel");
+        boxExtremaFinal<<<1, 64, 0, ctx->stream>>>(partial, blocks, partial + size_t(blocks) * 6);
+        SPHX_LAUNCH_CHECK(ctx, "boxExtremaFinal");
+        double e[6];
+        SPHX_HIP(ctx, hipMemcpyAsync(e, partial + size_t(blocks) * 6, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!keep[0]) r.xmin = e[0], r.xmax = e[3];
+        if (!keep[1]) r.ymin = e[1], r.ymax = e[4];
+        if (!keep[2]) r.zmin = e[2], r.zmax = e[5];
+    }
+    *out = r;
+    return SPHX_OK;
 }
 
 } // extern "C"

@@ -141,6 +141,15 @@ def kernel_tables(sinc_index: float = 6.0):
 
 
 # ------------------------------------------------------------------------------------------- SFC, sort
+def make_global_box(ctx: Context, x, y, z, previous: Box) -> Box:
+    """cstone::makeGlobalBox on one rank (sfc/box_mpi.hpp:68-107): open dimensions take
+    the coordinate extrema, periodic and fixed ones keep the limits of `previous`."""
+    out = Box()
+    ctx.check(ctx.lib.sphx_make_global_box(ctx.h, _ptr(x), _ptr(y), _ptr(z), x.numel(), C.byref(previous),
+                                           C.byref(out)))
+    return out
+
+
 def compute_sfc_keys(ctx: Context, x, y, z, box: Box, keys=None):
     n = x.numel()
     if keys is None:
@@ -440,8 +449,8 @@ def momentum_energy_adjoint(ctx: Context, d: ParticlesData, box: Box, g_ax=None,
 
 
 # ------------------------------------------------------------------------------------------- the step
-STAGES = ("keys", "sort", "gather", "octree", "link", "centers", "groups", "neighbors", "density", "eos", "iad",
-          "momentum_energy")
+STAGES = ("box", "keys", "sort", "gather", "octree", "link", "centers", "groups", "neighbors", "density", "eos",
+          "iad", "momentum_energy")
 
 
 class HydroStep:
@@ -472,36 +481,40 @@ class HydroStep:
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(STAGES) + 1)] if timed else None
         mark = (lambda k: ev[k].record(ctx.stream)) if timed else (lambda k: None)
         mark(0)
+        if not (box.bx and box.by and box.bz):
+            # open dimensions follow the particles (Domain::sync -> makeGlobalBox)
+            box = self.box = make_global_box(ctx, inp["x"], inp["y"], inp["z"], box)
+        mark(1)
         self.keys.zero_()
         compute_sfc_keys(ctx, inp["x"], inp["y"], inp["z"], box, self.keys)
-        mark(1)
+        mark(2)
         ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
         sort_by_key(ctx, self.keys, self.order)
-        mark(2)
+        mark(3)
         for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u"):
             gather(ctx, self.order, inp[f], getattr(d, f))
-        mark(3)
-        leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(4)
+        leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
+        mark(5)
         o = build_octree(ctx, leaves)
         o.layout, o.counts = layout, counts
-        mark(5)
+        mark(6)
         node_fp_centers(ctx, o, box)
         self.octree = o
-        mark(6)
+        mark(7)
         g = compute_groups(ctx, d.x, d.y, d.z, d.h, o, box) if self.spatial_groups else None
         self.groups = g
-        mark(7)
-        find_neighbors_sfc(ctx, d, box, o, g)
         mark(8)
-        compute_density(ctx, d, box, g)
+        find_neighbors_sfc(ctx, d, box, o, g)
         mark(9)
-        compute_eos_hydro_std(ctx, d)
+        compute_density(ctx, d, box, g)
         mark(10)
-        compute_iad(ctx, d, box, g)
+        compute_eos_hydro_std(ctx, d)
         mark(11)
-        compute_momentum_energy_std(ctx, d, box, g)
+        compute_iad(ctx, d, box, g)
         mark(12)
+        compute_momentum_energy_std(ctx, d, box, g)
+        mark(13)
         self.events = ev
         return d
 

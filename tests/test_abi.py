@@ -108,6 +108,7 @@ REFERENCE_TYPES = r"""
 #include "sphx_compat.hpp"
 void use(const cstone::OctreeNsView<double, uint64_t>& view, const cstone::GroupView& g) {
     cstone::Box<double> box(0, 1, cstone::BoundaryType::periodic);
+    box = sphx_compat::makeGlobalBox(nullptr, nullptr, nullptr, 0, box);
     sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
     sphx_compat::findNeighbors(nullptr, nullptr, nullptr, nullptr, 0, 0, box, view, 0, 150, nullptr, nullptr);
     sphx_compat::nodeFpCenters<cstone::Vec3<double>>(nullptr, 0, nullptr, nullptr, box);

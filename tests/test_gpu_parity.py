@@ -220,6 +220,9 @@ def test_hydro_step_vs_oracle(ctx, case, mode):
         inp, box = sedov_lattice(30)
     else:
         inp, box = uniform_box(20000, seed=21, periodic=(case == "uniform_periodic"))
+        if case == "uniform_open":
+            # the step fits open dimensions to the particles (makeGlobalBox); so does the oracle here
+            box = CPU.make_global_box(inp["x"], inp["y"], inp["z"], box)
     ref = oracle_step(inp, box)
     prec = MIXED if mode == "mixed" else FP64
     step, d = _run_gpu_step(ctx, inp, box, prec, exact=(mode == "fp64_exact"))
@@ -356,3 +359,20 @@ def test_compute_groups_reference_find_splits_known_answer(ctx):
     t.layout = layout
     g = hydro.compute_groups(ctx, dev(p), dev(p), dev(p), dev(h), t, hydro.make_box(0.0, 1.0, 0), tol)
     assert u32(g.bounds).tolist() == [0, 1, 3, 32, 34, 64]
+
+
+@pytest.mark.parametrize("bc", ["open", "mixed", "periodic"])
+def test_make_global_box_bit_exact(ctx, bc):
+    from sphexa_amd import hydro
+
+    print("case global box", bc, flush=True)
+    rng = np.random.default_rng(121)
+    n = 300_001
+    x, y, z = rng.normal(0, 3, n), rng.random(n) * 2 - 1, rng.standard_cauchy(n)
+    bcs = {"open": [oracle.BC_OPEN] * 3, "mixed": [oracle.BC_OPEN, oracle.BC_PERIODIC, oracle.BC_FIXED],
+           "periodic": [oracle.BC_PERIODIC] * 3}[bc]
+    prev = oracle.Box.make([-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], bcs)
+    ref = CPU.make_global_box(x, y, z, prev)
+    got = hydro.make_global_box(ctx, dev(x), dev(y), dev(z), hbox(prev))
+    for f in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "bx", "by", "bz"):
+        assert getattr(got, f) == getattr(ref, f), f

@@ -462,3 +462,18 @@ def test_groups_reference_find_splits_known_answer():
     b = CPU.compute_groups(p, p.copy(), p.copy(), h, _leaves_uniform(0), _layout_all_in_leaf0(64, 1),
                            Box.cube(0.0, 1.0), tol=tol)
     assert b.tolist() == [0, 1, 3, 32, 34, 64]
+
+
+# ----------------------------------------------------------------------------- global box
+def test_make_global_box_known_answer():
+    """makeGlobalBox: open dimensions take the coordinate extrema, periodic and fixed ones keep
+    their limits, n = 0 keeps the previous box (box_mpi.hpp:68-107)."""
+    x = np.array([0.3, -2.0, 5.5, 1.0])
+    y = np.array([7.0, 7.5, -1.0, 0.0])
+    z = np.array([0.1, 0.2, 0.3, 0.4])
+    prev = Box.make([0.0, 0.0, 0.0], [1.0, 2.0, 3.0], [oracle.BC_OPEN, oracle.BC_PERIODIC, oracle.BC_FIXED])
+    b = CPU.make_global_box(x, y, z, prev)
+    assert (b.xmin, b.xmax, b.ymin, b.ymax, b.zmin, b.zmax) == (-2.0, 5.5, 0.0, 2.0, 0.0, 3.0)
+    assert (b.bx, b.by, b.bz) == (oracle.BC_OPEN, oracle.BC_PERIODIC, oracle.BC_FIXED)
+    e = CPU.make_global_box(x[:0], y[:0], z[:0], prev)
+    assert (e.xmin, e.xmax) == (0.0, 1.0)

@@ -344,7 +344,7 @@ bool validBox(const sphx_box* b);
 // ---------------------------------------------------------------------------
 // wave64 helpers
 __device__ inline unsigned laneId() { return __lane_id(); }
-__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ inline unsigned popc64(uint64_t m) { return unsigned(__popcll(m)); }
 __device__ inline uint64_t lanemaskLt() { return (uint64_t(1) << laneId()) - 1; }
 

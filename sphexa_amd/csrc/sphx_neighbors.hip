@@ -6,11 +6,11 @@
 //                   dimensions); overlapping leaves are collected in LDS, sorted
 //                   (leaf order = SFC order = particle order) and merged into
 //                   contiguous particle ranges.
-//   nsCandidates  : the ranges are streamed in chunks of NS_CHUNK particles; each chunk is
+//   nsCandidates  : the ranges are streamed in chunks of 64 particles; each chunk is
 //                   filtered against the bounding boxes of the wave's four 16-target
-//                   quarters and compacted into per-quarter LDS lists, then lane
-//                   (quarter q, target t) tests quarter q's candidates against its
-//                   target with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
+//                   quarters and queued per quarter in LDS; a quarter's queue is tested
+//                   64 candidates at a time (lane = candidate, one target per step)
+//                   with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
 //                   separately rounded operations (findneighbors.hpp:33-60,115-140),
 //                   so the neighbor sets are bit-identical to the CPU path; hits are
 //                   appended in ascending j, which is also the CPU's depth-first
@@ -19,6 +19,7 @@
 // re-runs only the groups that still have particles outside [ng0/4, ngmax + 1].
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "sphx_internal.hpp"
 
@@ -27,7 +28,6 @@ namespace sphx
 
 constexpr int NS_FCAP = 2048; // BFS frontier capacity (per buffer)
 constexpr int NS_LCAP = 1024; // candidate leaves per group
-constexpr uint32_t NS_CHUNK = 32; // candidates staged per step of nsCandidates
 
 struct NsFlags
 {
@@ -235,12 +235,26 @@ __device__ inline double updateH(unsigned ng0, unsigned nc, double h)
 // SPH list format, per sub-group of 16 consecutive targets (quarter q of the wave,
 // sg = 4 g + q): the sorted union of the stored neighbors of its 16 targets
 // (unions[sg][0..usize)) and per target a bitmask over that union
-// (masks[sg][w][t]).  Each quarter's candidates arrive in ascending j, so a union
-// entry is appended the first time any target of the quarter hits it; the lanes of
-// the quarter build their mask words in registers and store one coalesced 64-byte
-// row per 32 union entries.
+// (masks[sg][w][t]).
+//
+// Thread mapping (transposed test).  The group's candidate ranges are streamed in
+// chunks of 64 particles, lane = candidate; every chunk is filtered against the
+// bounding boxes of the four quarters (16 targets each) and the survivors are
+// appended, in ascending j, to per-quarter LDS queues.  A queue holding 64 candidates
+// (or the rest, at the end) is tested as one batch: lane = candidate, and the wave
+// walks the quarter's 16 targets, one target per step, so a step costs a handful of
+// FP64 instructions for 64 tests and its ballot is the target's hit mask over the
+// batch.  The hit masks give the batch's union entries (candidates stored by some
+// target, in lane order = ascending j); a lane permutation compacts the entries and
+// one ballot per target turns the compacted bits into that target's mask bits, which
+// the target's lane appends to its mask word stream.
+constexpr uint32_t NS_QCAP = 128; // per-quarter queue capacity (ring buffer)
+
+// v_writelane_b32: `old` with lane `lane` replaced by the wave-uniform value v
+__device__ int writeLane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 template<class Th, bool Sph>
-__global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                    const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, Th* __restrict__ h,
                                                    uint8_t* __restrict__ iter, const uint2* __restrict__ ranges,
@@ -250,11 +264,10 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                                                    NsFlags* __restrict__ flags)
 {
 #pragma clang fp contract(off)
-    // per quarter: the staged candidates that survive the quarter's box filter.  Chunks
-    // of 32 candidates keep LDS at 3.7 KB per wave (measured faster than 64: occupancy)
-    __shared__ double   cx[4][NS_CHUNK], cy[4][NS_CHUNK], cz[4][NS_CHUNK];
-    __shared__ uint32_t cj[4][NS_CHUNK];
-    __shared__ double   qbox[4][7]; // centre xyz, half-width xyz, periodic flag
+    __shared__ uint32_t qj[4][NS_QCAP]; // queued candidate indices per quarter
+    __shared__ double   tgt[64][4];     // target of lane l: x, y, z, (2h)^2
+    __shared__ uint32_t tid[64];        // its particle index
+    __shared__ double   qbox[4][7];     // quarter box: centre xyz, half-width xyz, periodic flag
 
     const uint32_t blk  = xcdContiguous(blockIdx.x, gridDim.x);
     const uint32_t g    = groupList ? groupList[blk] : blk;
@@ -285,6 +298,8 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
     // a neighbor of any target of the quarter lies inside it (its image nearest the
     // box centre, when the quarter uses periodic images)
     const unsigned q = lane >> 4, t = lane & 15u;
+    const uint64_t actMask = ballot(act);
+    const uint64_t pbcMask = ballot(usePbc);
     {
         double pp[3] = {xi, yi, zi};
         double r     = 2.0 * hi;
@@ -301,7 +316,6 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
             }
             lo[d] = a, hh[d] = b;
         }
-        uint64_t pbcMask = ballot(usePbc);
         if (t == 0)
         {
 #pragma unroll
@@ -313,37 +327,44 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
             }
             qbox[q][6] = ((pbcMask >> (16u * q)) & 0xFFFFu) ? 1.0 : 0.0;
         }
+        tgt[lane][0] = xi, tgt[lane][1] = yi, tgt[lane][2] = zi, tgt[lane][3] = radiusSq;
+        tid[lane]    = act ? i : 0xFFFFFFFFu;
     }
     __syncthreads();
 
-    // non-SPH: reference CPU layout nbrs[(i - first) * ngmax + k]
-    const size_t lbase = size_t(s + lane - G.first) * ngmax;
-    // SPH: this lane's sub-group
-    const uint32_t sg     = 4u * g + q;
-    uint32_t*      myMask = Sph ? out.masks + size_t(sg) * out.words * 16 + t : nullptr;
-    uint32_t*      myUni  = Sph ? out.unions + size_t(sg) * out.capU : nullptr;
-    uint32_t       usz    = 0;
-    uint32_t       word   = 0; // mask bits of union entries 32 * (usz / 32) ...
-    uint32_t       count  = 0;
-    const uint64_t lt     = lanemaskLt();
-    const uint32_t qAct   = uint32_t(popc64((ballot(act) >> (16u * q)) & 0xFFFFu));
+    const uint64_t lt = lanemaskLt();
+    // per-target state, in the target's lane
+    uint32_t count = 0; // hits so far (stored or not)
+    uint32_t word  = 0; // SPH: mask bits of union entries 32 * (usz / 32) ...
+    uint32_t extra = 0; // hits of the current batch beyond the per-target cap (SPH)
+    uint32_t delta = 0; // hits of the current batch (non-SPH)
+    // per-quarter state (wave-uniform)
+    uint32_t qhead[4] = {0, 0, 0, 0}, qlen[4] = {0, 0, 0, 0}, usz[4] = {0, 0, 0, 0};
     unsigned long long tests = 0;
 
-    const uint32_t R = rangeCount[g];
-    for (uint32_t r = 0; r < R; ++r)
+    const uint32_t R   = rangeCount[g];
+    uint32_t       r   = 0;
+    uint2          rg  = R ? ranges[size_t(g) * NS_LCAP] : make_uint2(0u, 0u);
+    uint32_t       cur = rg.x;
+    for (;;)
     {
-        uint2 rg = ranges[size_t(g) * NS_LCAP + r];
-        for (uint32_t base = rg.x; base < rg.y; base += NS_CHUNK)
-        {
-            // stage: lane = candidate; filter against the 4 quarter boxes; compact
-            uint32_t m  = min(NS_CHUNK, rg.y - base);
-            bool     in = lane < m;
-            uint32_t j  = base + lane;
-            double   ax = 0, ay = 0, az = 0;
-            if (in) ax = x[j], ay = y[j], az = z[j];
-            uint32_t num[4];
+        // a full queue is tested first; else the next chunk is staged; at the end the
+        // remaining partial queues are tested
+        uint32_t bq = 4, nb = 0;
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq)
+        for (uint32_t qq = 0; qq < 4; ++qq)
+            if (bq == 4 && qlen[qq] >= 64) bq = qq, nb = 64;
+        const bool more = r < R;
+        if (bq == 4 && more)
+        {
+            // stage: lane = candidate; filter against the 4 quarter boxes; queue
+            const uint32_t m  = min(64u, rg.y - cur);
+            const bool     in = lane < m;
+            const uint32_t j  = cur + lane;
+            double         ax = 0, ay = 0, az = 0;
+            if (in) ax = x[j], ay = y[j], az = z[j];
+#pragma unroll
+            for (uint32_t qq = 0; qq < 4; ++qq)
             {
                 double dx = ax - qbox[qq][0], dy = ay - qbox[qq][1], dz = az - qbox[qq][2];
                 if (qbox[qq][6] != 0.0)
@@ -354,68 +375,173 @@ __global__ __launch_bounds__(64) void nsCandidates(GroupsDev G, const uint32_t* 
                 }
                 bool     pass = in && fabs(dx) <= qbox[qq][3] && fabs(dy) <= qbox[qq][4] && fabs(dz) <= qbox[qq][5];
                 uint64_t mk   = ballot(pass);
-                if (pass)
-                {
-                    uint32_t pos = uint32_t(popc64(mk & lt));
-                    cx[qq][pos] = ax, cy[qq][pos] = ay, cz[qq][pos] = az, cj[qq][pos] = j;
-                }
-                num[qq] = uint32_t(popc64(mk));
+                if (pass) qj[qq][(qhead[qq] + qlen[qq] + popc64(mk & lt)) & (NS_QCAP - 1)] = j;
+                qlen[qq] += popc64(mk);
+            }
+            cur += m;
+            if (cur >= rg.y && ++r < R)
+            {
+                rg  = ranges[size_t(g) * NS_LCAP + r];
+                cur = rg.x;
             }
             __syncthreads();
-            const uint32_t myNum = num[q];
-            const uint32_t steps = max(max(num[0], num[1]), max(num[2], num[3]));
-            tests += (unsigned long long)myNum * qAct;
-            // test: lane (q, t) takes candidate k of quarter q's list against target t
-            for (uint32_t k = 0; k < steps; ++k)
+            continue;
+        }
+        if (bq == 4)
+        {
+#pragma unroll
+            for (uint32_t qq = 0; qq < 4; ++qq)
+                if (bq == 4 && qlen[qq] > 0) bq = qq, nb = qlen[qq];
+            if (bq == 4) break;
+        }
+
+        // ---- test batch: quarter bq, nb queued candidates, lane = candidate
+        const uint32_t qb    = 16u * bq;
+        const bool     valid = lane < nb;
+        uint32_t       jc    = 0;
+        double         xc = 0, yc = 0, zc = 0;
+        if (valid)
+        {
+            jc = qj[bq][(qhead[bq] + lane) & (NS_QCAP - 1)];
+            xc = x[jc], yc = y[jc], zc = z[jc];
+        }
+        const uint32_t qAct = uint32_t((actMask >> qb) & 0xFFFFu);
+        tests += (unsigned long long)nb * popc64(qAct);
+        uint32_t hb = 0; // bit tt: target tt of the quarter hits this candidate
+        // straight-line steps; an inactive target has (2h)^2 = 0 and never hits
+        if (((pbcMask >> qb) & 0xFFFFu) == 0)
+        {
+#pragma unroll
+            for (uint32_t tt = 0; tt < 16; ++tt)
             {
-                bool   valid = k < myNum;
-                uint32_t kk  = valid ? k : 0u;
-                double dx  = cx[q][kk] - xi;
-                double dy  = cy[q][kk] - yi;
-                double dz  = cz[q][kk] - zi;
-                if (usePbc)
+                const uint32_t tl = qb + tt;
+                const double   dx = xc - tgt[tl][0];
+                const double   dy = yc - tgt[tl][1];
+                const double   dz = zc - tgt[tl][2];
+                const double   d2 = dx * dx + dy * dy + dz * dz;
+                hb |= (valid & (d2 < tgt[tl][3])) ? (1u << tt) : 0u;
+            }
+        }
+        else
+        {
+            // quarters with periodic targets (near the box faces): the targets that use
+            // periodic images fold the offsets (volatile reads keep this rare path's loads
+            // from being merged with the common path)
+            const volatile double* tv = &tgt[0][0];
+#pragma unroll 4
+            for (uint32_t tt = 0; tt < 16; ++tt)
+            {
+                const uint32_t tl = qb + tt;
+                double         dx = xc - tv[4 * tl + 0];
+                double         dy = yc - tv[4 * tl + 1];
+                double         dz = zc - tv[4 * tl + 2];
+                if ((pbcMask >> tl) & 1u)
                 {
                     dx -= px * rint(dx * box.ilen[0]);
                     dy -= py * rint(dy * box.ilen[1]);
                     dz -= pz * rint(dz * box.ilen[2]);
                 }
-                double   d2  = dx * dx + dy * dy + dz * dz;
-                uint32_t jj  = cj[q][kk];
-                bool     hit = act && valid && d2 < radiusSq && jj != i;
-                if constexpr (Sph)
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                hb |= (valid & (d2 < tv[4 * tl + 3])) ? (1u << tt) : 0u;
+            }
+        }
+        // a target is not its own neighbor
+        if (valid && jc - (s + qb) < 16u) hb &= ~(1u << (jc - (s + qb)));
+
+        const bool mine = q == bq;
+        if constexpr (Sph)
+        {
+            // union entries of the batch (candidates stored by some target) compacted into
+            // lanes 0..nU-1 by a lane permutation; target lane qb + tt receives its mask
+            // bits over them in (cmHi:cmLo)
+            uint32_t nU = 0, jp = 0, cmLo = 0, cmHi = 0;
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                const uint64_t U = ballot(hb != 0);
+                nU               = popc64(U);
+                cmLo = cmHi = 0;
+                if (nU)
                 {
-                    bool     store = hit && count < ngmax;
-                    uint32_t mq    = uint32_t(ballot(store) >> (16u * q)) & 0xFFFFu;
-                    if (mq)
+                    const bool     inU = (U >> lane) & 1u;
+                    const uint32_t dst = inU ? popc64(U & lt) : nU + popc64(~U & lt);
+                    const uint32_t hp  = uint32_t(__builtin_amdgcn_ds_permute(int(4u * dst), int(hb)));
+                    jp                 = uint32_t(__builtin_amdgcn_ds_permute(int(4u * dst), int(jc)));
+#pragma unroll
+                    for (uint32_t tt = 0; tt < 16; ++tt)
                     {
-                        if (store) word |= 1u << (usz & 31u);
-                        if (t == 0) myUni[usz] = jj;
-                        ++usz;
-                        if ((usz & 31u) == 0)
-                        {
-                            myMask[size_t(usz / 32u - 1u) * 16] = word;
-                            word = 0;
-                        }
+                        const uint64_t cm = ballot((hp >> tt) & 1u);
+                        cmLo = uint32_t(writeLane(int(uint32_t(cm)), int(qb + tt), int(cmLo)));
+                        cmHi = uint32_t(writeLane(int(uint32_t(cm >> 32)), int(qb + tt), int(cmHi)));
                     }
                 }
-                else
+                const uint32_t gain = uint32_t(__builtin_popcount(cmLo)) + uint32_t(__builtin_popcount(cmHi));
+                if (pass == 1 || ballot(mine && count + gain > ngmax) == 0) break;
+                // a target reaches its cap in this batch: keep only its first hits, count
+                // the others, and compact again
+#pragma unroll 1
+                for (uint32_t tt = 0; tt < 16; ++tt)
                 {
-                    if (hit && nbrs && count < ngmax) nbrs[lbase + count] = jj;
+                    const bool     hit  = (hb >> tt) & 1u;
+                    const uint64_t mh   = ballot(hit);
+                    const uint32_t pc   = popc64(mh);
+                    const uint32_t ct   = uint32_t(__builtin_amdgcn_readlane(int(count), int(qb + tt)));
+                    const uint32_t room = ct < ngmax ? ngmax - ct : 0u;
+                    if (pc > room)
+                    {
+                        if (hit && popc64(mh & lt) >= room) hb &= ~(1u << tt);
+                        extra = uint32_t(writeLane(int(pc - room), int(qb + tt), int(extra)));
+                    }
                 }
-                count += hit ? 1u : 0u;
             }
-            __syncthreads();
+            if (lane < nU) out.unions[size_t(4u * g + bq) * out.capU + usz[bq] + lane] = jp;
+            if (mine)
+            {
+                count += uint32_t(__builtin_popcount(cmLo)) + uint32_t(__builtin_popcount(cmHi)) + extra;
+                extra = 0;
+                if (nU)
+                {
+                    // append nU mask bits at bit offset usz of the target's word stream
+                    const uint32_t u0 = usz[bq], o = u0 & 31u, end = o + nU;
+                    const uint64_t cm64 = (uint64_t(cmHi) << 32) | cmLo;
+                    const uint64_t lo64 = uint64_t(word) | (cm64 << o);
+                    const uint32_t w2   = o ? uint32_t(cm64 >> (64u - o)) : 0u;
+                    uint32_t* wp = out.masks + (size_t(4u * g + bq) * out.words + (u0 >> 5)) * 16 + t;
+                    if (end >= 32) wp[0] = uint32_t(lo64);
+                    if (end >= 64) wp[16] = uint32_t(lo64 >> 32);
+                    word = end >= 64 ? w2 : (end >= 32 ? uint32_t(lo64 >> 32) : uint32_t(lo64));
+                }
+            }
+            usz[bq] += nU;
         }
+        else
+        {
+            // reference CPU layout: the k-th hit of target i goes to nbrs[(i - first) * ngmax + k]
+#pragma unroll 1
+            for (uint32_t tt = 0; tt < 16; ++tt)
+            {
+                const bool     hit = (hb >> tt) & 1u;
+                const uint64_t mh  = ballot(hit);
+                if (mh == 0) continue;
+                const uint32_t ct  = uint32_t(__builtin_amdgcn_readlane(int(count), int(qb + tt)));
+                const uint32_t pos = ct + popc64(mh & lt);
+                if (hit && nbrs && pos < ngmax) nbrs[size_t(s + qb + tt - G.first) * ngmax + pos] = jc;
+                delta = uint32_t(writeLane(int(popc64(mh)), int(qb + tt), int(delta)));
+            }
+            if (mine) count += delta;
+            delta = 0;
+        }
+        qhead[bq] += nb;
+        qlen[bq] -= nb;
     }
-    // exact pair tests of this pass, per quarter (each quarter's count once)
-    tests = (t == 0) ? tests : 0ull;
-    tests = waveSum(tests);
+    // exact pair tests of this pass
     if (lane == 0 && tests) atomicAdd(&flags->tests, tests);
 
     if constexpr (Sph)
     {
-        if (t == 0) out.usize[sg] = usz;
-        if (usz & 31u) myMask[size_t(usz / 32u) * 16] = word;
+        const uint32_t sg    = 4u * g + q;
+        const uint32_t myUsz = q == 0 ? usz[0] : q == 1 ? usz[1] : q == 2 ? usz[2] : usz[3];
+        if (t == 0) out.usize[sg] = myUsz;
+        if (myUsz & 31u) out.masks[(size_t(sg) * out.words + myUsz / 32u) * 16 + t] = word;
         bool redo = false;
         if (act)
         {

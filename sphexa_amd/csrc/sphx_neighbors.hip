@@ -253,6 +253,12 @@ constexpr uint32_t NS_QCAP = 128; // per-quarter queue capacity (ring buffer)
 // v_writelane_b32: `old` with lane `lane` replaced by the wave-uniform value v
 __device__ int writeLane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
+// number of set bits of m below this lane (v_mbcnt)
+__device__ inline uint32_t rankIn(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
 template<class Th, bool Sph>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                    const double* __restrict__ x, const double* __restrict__ y,
@@ -338,75 +344,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
     uint32_t word  = 0; // SPH: mask bits of union entries 32 * (usz / 32) ...
     uint32_t extra = 0; // hits of the current batch beyond the per-target cap (SPH)
     uint32_t delta = 0; // hits of the current batch (non-SPH)
-    // per-quarter state (wave-uniform)
+    // per-quarter state (wave-uniform; indexed by compile-time quarter numbers only)
     uint32_t qhead[4] = {0, 0, 0, 0}, qlen[4] = {0, 0, 0, 0}, usz[4] = {0, 0, 0, 0};
     unsigned long long tests = 0;
 
-    const uint32_t R   = rangeCount[g];
-    uint32_t       r   = 0;
-    uint2          rg  = R ? ranges[size_t(g) * NS_LCAP] : make_uint2(0u, 0u);
-    uint32_t       cur = rg.x;
-    for (;;)
-    {
-        // a full queue is tested first; else the next chunk is staged; at the end the
-        // remaining partial queues are tested
-        uint32_t bq = 4, nb = 0;
-#pragma unroll
-        for (uint32_t qq = 0; qq < 4; ++qq)
-            if (bq == 4 && qlen[qq] >= 64) bq = qq, nb = 64;
-        const bool more = r < R;
-        if (bq == 4 && more)
-        {
-            // stage: lane = candidate; filter against the 4 quarter boxes; queue
-            const uint32_t m  = min(64u, rg.y - cur);
-            const bool     in = lane < m;
-            const uint32_t j  = cur + lane;
-            double         ax = 0, ay = 0, az = 0;
-            if (in) ax = x[j], ay = y[j], az = z[j];
-#pragma unroll
-            for (uint32_t qq = 0; qq < 4; ++qq)
-            {
-                double dx = ax - qbox[qq][0], dy = ay - qbox[qq][1], dz = az - qbox[qq][2];
-                if (qbox[qq][6] != 0.0)
-                {
-                    dx -= px * rint(dx * box.ilen[0]);
-                    dy -= py * rint(dy * box.ilen[1]);
-                    dz -= pz * rint(dz * box.ilen[2]);
-                }
-                bool     pass = in && fabs(dx) <= qbox[qq][3] && fabs(dy) <= qbox[qq][4] && fabs(dz) <= qbox[qq][5];
-                uint64_t mk   = ballot(pass);
-                if (pass) qj[qq][(qhead[qq] + qlen[qq] + popc64(mk & lt)) & (NS_QCAP - 1)] = j;
-                qlen[qq] += popc64(mk);
-            }
-            cur += m;
-            if (cur >= rg.y && ++r < R)
-            {
-                rg  = ranges[size_t(g) * NS_LCAP + r];
-                cur = rg.x;
-            }
-            __syncthreads();
-            continue;
-        }
-        if (bq == 4)
-        {
-#pragma unroll
-            for (uint32_t qq = 0; qq < 4; ++qq)
-                if (bq == 4 && qlen[qq] > 0) bq = qq, nb = qlen[qq];
-            if (bq == 4) break;
-        }
-
-        // ---- test batch: quarter bq, nb queued candidates, lane = candidate
-        const uint32_t qb    = 16u * bq;
-        const bool     valid = lane < nb;
-        uint32_t       jc    = 0;
-        double         xc = 0, yc = 0, zc = 0;
+    // test the first nb queued candidates of quarter Q: lane = candidate
+    auto batch = [&](auto Qc, uint32_t nb) {
+        constexpr uint32_t Q  = decltype(Qc)::value;
+        constexpr uint32_t qb = 16u * Q;
+        const bool         valid = lane < nb;
+        uint32_t           jc    = 0;
+        double             xc = 0, yc = 0, zc = 0;
         if (valid)
         {
-            jc = qj[bq][(qhead[bq] + lane) & (NS_QCAP - 1)];
+            jc = qj[Q][(qhead[Q] + lane) & (NS_QCAP - 1)];
             xc = x[jc], yc = y[jc], zc = z[jc];
         }
-        const uint32_t qAct = uint32_t((actMask >> qb) & 0xFFFFu);
-        tests += (unsigned long long)nb * popc64(qAct);
+        tests += (unsigned long long)nb * popc64((actMask >> qb) & 0xFFFFu);
         uint32_t hb = 0; // bit tt: target tt of the quarter hits this candidate
         // straight-line steps; an inactive target has (2h)^2 = 0 and never hits
         if (((pbcMask >> qb) & 0xFFFFu) == 0)
@@ -414,12 +368,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
 #pragma unroll
             for (uint32_t tt = 0; tt < 16; ++tt)
             {
-                const uint32_t tl = qb + tt;
-                const double   dx = xc - tgt[tl][0];
-                const double   dy = yc - tgt[tl][1];
-                const double   dz = zc - tgt[tl][2];
-                const double   d2 = dx * dx + dy * dy + dz * dz;
-                hb |= (valid & (d2 < tgt[tl][3])) ? (1u << tt) : 0u;
+                const double dx = xc - tgt[qb + tt][0];
+                const double dy = yc - tgt[qb + tt][1];
+                const double dz = zc - tgt[qb + tt][2];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                hb |= (valid & (d2 < tgt[qb + tt][3])) ? (1u << tt) : 0u;
             }
         }
         else
@@ -427,28 +380,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
             // quarters with periodic targets (near the box faces): the targets that use
             // periodic images fold the offsets (volatile reads keep this rare path's loads
             // from being merged with the common path)
-            const volatile double* tv = &tgt[0][0];
+            const volatile double* tv = &tgt[qb][0];
 #pragma unroll 4
             for (uint32_t tt = 0; tt < 16; ++tt)
             {
-                const uint32_t tl = qb + tt;
-                double         dx = xc - tv[4 * tl + 0];
-                double         dy = yc - tv[4 * tl + 1];
-                double         dz = zc - tv[4 * tl + 2];
-                if ((pbcMask >> tl) & 1u)
+                double dx = xc - tv[4 * tt + 0];
+                double dy = yc - tv[4 * tt + 1];
+                double dz = zc - tv[4 * tt + 2];
+                if ((pbcMask >> (qb + tt)) & 1u)
                 {
                     dx -= px * rint(dx * box.ilen[0]);
                     dy -= py * rint(dy * box.ilen[1]);
                     dz -= pz * rint(dz * box.ilen[2]);
                 }
                 const double d2 = dx * dx + dy * dy + dz * dz;
-                hb |= (valid & (d2 < tv[4 * tl + 3])) ? (1u << tt) : 0u;
+                hb |= (valid & (d2 < tv[4 * tt + 3])) ? (1u << tt) : 0u;
             }
         }
         // a target is not its own neighbor
-        if (valid && jc - (s + qb) < 16u) hb &= ~(1u << (jc - (s + qb)));
+        {
+            const uint32_t self = jc - (s + qb);
+            if (valid & (self < 16u)) hb &= ~(1u << self);
+        }
 
-        const bool mine = q == bq;
+        const bool mine = q == Q;
         if constexpr (Sph)
         {
             // union entries of the batch (candidates stored by some target) compacted into
@@ -462,8 +417,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                 cmLo = cmHi = 0;
                 if (nU)
                 {
-                    const bool     inU = (U >> lane) & 1u;
-                    const uint32_t dst = inU ? popc64(U & lt) : nU + popc64(~U & lt);
+                    const uint32_t dst = hb ? rankIn(U) : nU + rankIn(~U);
                     const uint32_t hp  = uint32_t(__builtin_amdgcn_ds_permute(int(4u * dst), int(hb)));
                     jp                 = uint32_t(__builtin_amdgcn_ds_permute(int(4u * dst), int(jc)));
 #pragma unroll
@@ -488,12 +442,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                     const uint32_t room = ct < ngmax ? ngmax - ct : 0u;
                     if (pc > room)
                     {
-                        if (hit && popc64(mh & lt) >= room) hb &= ~(1u << tt);
+                        if (hit && rankIn(mh) >= room) hb &= ~(1u << tt);
                         extra = uint32_t(writeLane(int(pc - room), int(qb + tt), int(extra)));
                     }
                 }
             }
-            if (lane < nU) out.unions[size_t(4u * g + bq) * out.capU + usz[bq] + lane] = jp;
+            if (lane < nU) out.unions[size_t(4u * g + Q) * out.capU + usz[Q] + lane] = jp;
             if (mine)
             {
                 count += uint32_t(__builtin_popcount(cmLo)) + uint32_t(__builtin_popcount(cmHi)) + extra;
@@ -501,17 +455,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                 if (nU)
                 {
                     // append nU mask bits at bit offset usz of the target's word stream
-                    const uint32_t u0 = usz[bq], o = u0 & 31u, end = o + nU;
+                    const uint32_t u0 = usz[Q], o = u0 & 31u, end = o + nU;
                     const uint64_t cm64 = (uint64_t(cmHi) << 32) | cmLo;
                     const uint64_t lo64 = uint64_t(word) | (cm64 << o);
                     const uint32_t w2   = o ? uint32_t(cm64 >> (64u - o)) : 0u;
-                    uint32_t* wp = out.masks + (size_t(4u * g + bq) * out.words + (u0 >> 5)) * 16 + t;
+                    uint32_t* wp = out.masks + (size_t(4u * g + Q) * out.words + (u0 >> 5)) * 16 + t;
                     if (end >= 32) wp[0] = uint32_t(lo64);
                     if (end >= 64) wp[16] = uint32_t(lo64 >> 32);
                     word = end >= 64 ? w2 : (end >= 32 ? uint32_t(lo64 >> 32) : uint32_t(lo64));
                 }
             }
-            usz[bq] += nU;
+            usz[Q] += nU;
         }
         else
         {
@@ -523,15 +477,64 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                 const uint64_t mh  = ballot(hit);
                 if (mh == 0) continue;
                 const uint32_t ct  = uint32_t(__builtin_amdgcn_readlane(int(count), int(qb + tt)));
-                const uint32_t pos = ct + popc64(mh & lt);
+                const uint32_t pos = ct + rankIn(mh);
                 if (hit && nbrs && pos < ngmax) nbrs[size_t(s + qb + tt - G.first) * ngmax + pos] = jc;
                 delta = uint32_t(writeLane(int(popc64(mh)), int(qb + tt), int(delta)));
             }
             if (mine) count += delta;
             delta = 0;
         }
-        qhead[bq] += nb;
-        qlen[bq] -= nb;
+        qhead[Q] += nb;
+        qlen[Q] -= nb;
+    };
+
+    // Stream the candidates in chunks of 64 (lane = candidate): filter each chunk against
+    // the 4 quarter boxes and queue the survivors; then every quarter whose queue holds
+    // 64 candidates tests one batch (a chunk adds at most 64, so queues stay below 128).
+    // After the last chunk the remaining partial queues are tested.
+    const uint32_t R   = rangeCount[g];
+    uint32_t       r   = 0;
+    uint2          rg  = R ? ranges[size_t(g) * NS_LCAP] : make_uint2(0u, 0u);
+    uint32_t       cur = rg.x;
+    for (;;)
+    {
+        const bool done = r >= R;
+        if (!done)
+        {
+            const uint32_t m  = min(64u, rg.y - cur);
+            const bool     in = lane < m;
+            const uint32_t j  = cur + lane;
+            double         ax = 0, ay = 0, az = 0;
+            if (in) ax = x[j], ay = y[j], az = z[j];
+#pragma unroll
+            for (uint32_t qq = 0; qq < 4; ++qq)
+            {
+                double dx = ax - qbox[qq][0], dy = ay - qbox[qq][1], dz = az - qbox[qq][2];
+                if ((pbcMask >> (16u * qq)) & 0xFFFFu)
+                {
+                    dx -= px * rint(dx * box.ilen[0]);
+                    dy -= py * rint(dy * box.ilen[1]);
+                    dz -= pz * rint(dz * box.ilen[2]);
+                }
+                const bool pass = in & (fabs(dx) <= qbox[qq][3]) & (fabs(dy) <= qbox[qq][4]) &
+                                  (fabs(dz) <= qbox[qq][5]);
+                const uint64_t mk = ballot(pass);
+                if (pass) qj[qq][(qhead[qq] + qlen[qq] + rankIn(mk)) & (NS_QCAP - 1)] = j;
+                qlen[qq] += popc64(mk);
+            }
+            cur += m;
+            if (cur >= rg.y && ++r < R)
+            {
+                rg  = ranges[size_t(g) * NS_LCAP + r];
+                cur = rg.x;
+            }
+            __syncthreads();
+        }
+        if (qlen[0] >= 64 || (done && qlen[0])) batch(std::integral_constant<uint32_t, 0>{}, min(qlen[0], 64u));
+        if (qlen[1] >= 64 || (done && qlen[1])) batch(std::integral_constant<uint32_t, 1>{}, min(qlen[1], 64u));
+        if (qlen[2] >= 64 || (done && qlen[2])) batch(std::integral_constant<uint32_t, 2>{}, min(qlen[2], 64u));
+        if (qlen[3] >= 64 || (done && qlen[3])) batch(std::integral_constant<uint32_t, 3>{}, min(qlen[3], 64u));
+        if (done) break;
     }
     // exact pair tests of this pass
     if (lane == 0 && tests) atomicAdd(&flags->tests, tests);

@@ -7,8 +7,9 @@
 //                   (leaf order = SFC order = particle order) and merged into
 //                   contiguous particle ranges.
 //   nsCandidates  : the ranges are streamed in chunks of 64 particles; each chunk is
-//                   filtered against the bounding boxes of the wave's four 16-target
-//                   quarters and queued per quarter in LDS; a quarter's queue is tested
+//                   filtered against the regions of the wave's four 16-target quarters
+//                   (points within the largest 2h of the targets' bounding box) and
+//                   queued per quarter in LDS; a quarter's queue is tested
 //                   64 candidates at a time (lane = candidate, one target per step)
 //                   with the reference predicate d^2 < (2h)^2 evaluated in FP64 with
 //                   separately rounded operations (findneighbors.hpp:33-60,115-140),
@@ -239,8 +240,8 @@ __device__ inline double updateH(unsigned ng0, unsigned nc, double h)
 //
 // Thread mapping (transposed test).  The group's candidate ranges are streamed in
 // chunks of 64 particles, lane = candidate; every chunk is filtered against the
-// bounding boxes of the four quarters (16 targets each) and the survivors are
-// appended, in ascending j, to per-quarter LDS queues.  A queue holding 64 candidates
+// regions of the four quarters (16 targets each) and the survivors are appended, in
+// ascending j, to per-quarter LDS queues.  A queue holding 64 candidates
 // (or the rest, at the end) is tested as one batch: lane = candidate, and the wave
 // walks the quarter's 16 targets, one target per step, so a step costs a handful of
 // FP64 instructions for 64 tests and its ballot is the target's hit mask over the
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
     __shared__ uint32_t qj[4][NS_QCAP]; // queued candidate indices per quarter
     __shared__ double   tgt[64][4];     // target of lane l: x, y, z, (2h)^2
     __shared__ uint32_t tid[64];        // its particle index
-    __shared__ double   qbox[4][7];     // quarter box: centre xyz, half-width xyz, periodic flag
+    __shared__ double   qbox[4][7];     // quarter: box centre xyz, half-widths xyz, (radius)^2
 
     const uint32_t blk  = xcdContiguous(blockIdx.x, gridDim.x);
     const uint32_t g    = groupList ? groupList[blk] : blk;
@@ -299,21 +300,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
     const double py     = box.pbc[1] ? box.len[1] : 0.0;
     const double pz     = box.pbc[2] ? box.len[2] : 0.0;
 
-    // quarter boxes: the bounding box of the 16 targets' 2h spheres, widened by a
-    // relative 1e-12 so that rounding never drops a candidate; a candidate that is
-    // a neighbor of any target of the quarter lies inside it (its image nearest the
-    // box centre, when the quarter uses periodic images)
+    // quarter regions: the bounding box of the 16 targets' positions (centre, half-widths
+    // widened by a relative 1e-12) and the largest search radius 2h of the quarter.  A
+    // neighbor of any target of the quarter lies within that radius of the box (its image
+    // nearest the box centre, when the quarter uses periodic images), so the filter keeps
+    // candidates whose distance to the box is at most the radius, with a relative slack of
+    // 1e-9 on the squared radius; a quarter without targets keeps nothing.
     const unsigned q = lane >> 4, t = lane & 15u;
     const uint64_t actMask = ballot(act);
     const uint64_t pbcMask = ballot(usePbc);
     {
         double pp[3] = {xi, yi, zi};
-        double r     = 2.0 * hi;
         double lo[3], hh[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
         {
-            double a = act ? pp[d] - r : INFINITY, b = act ? pp[d] + r : -INFINITY;
+            double a = act ? pp[d] : INFINITY, b = act ? pp[d] : -INFINITY;
 #pragma unroll
             for (int o = 1; o < 16; o <<= 1)
             {
@@ -322,6 +324,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
             }
             lo[d] = a, hh[d] = b;
         }
+        double rmax = act ? 2.0 * hi : -1.0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1)
+            rmax = fmax(rmax, __shfl_xor(rmax, o, 64));
         if (t == 0)
         {
 #pragma unroll
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                 qbox[q][d]     = c;
                 qbox[q][3 + d] = w + 1e-12 * (fabs(c) + w + box.len[d]);
             }
-            qbox[q][6] = ((pbcMask >> (16u * q)) & 0xFFFFu) ? 1.0 : 0.0;
+            qbox[q][6] = rmax < 0.0 ? -1.0 : rmax * rmax * (1.0 + 1e-9);
         }
         tgt[lane][0] = xi, tgt[lane][1] = yi, tgt[lane][2] = zi, tgt[lane][3] = radiusSq;
         tid[lane]    = act ? i : 0xFFFFFFFFu;
@@ -516,8 +522,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                     dy -= py * rint(dy * box.ilen[1]);
                     dz -= pz * rint(dz * box.ilen[2]);
                 }
-                const bool pass = in & (fabs(dx) <= qbox[qq][3]) & (fabs(dy) <= qbox[qq][4]) &
-                                  (fabs(dz) <= qbox[qq][5]);
+                const double ex   = fmax(fabs(dx) - qbox[qq][3], 0.0);
+                const double ey   = fmax(fabs(dy) - qbox[qq][4], 0.0);
+                const double ez   = fmax(fabs(dz) - qbox[qq][5], 0.0);
+                const bool   pass = in & (ex * ex + ey * ey + ez * ez <= qbox[qq][6]);
                 const uint64_t mk = ballot(pass);
                 if (pass) qj[qq][(qhead[qq] + qlen[qq] + rankIn(mk)) & (NS_QCAP - 1)] = j;
                 qlen[qq] += popc64(mk);

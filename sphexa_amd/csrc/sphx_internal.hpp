@@ -347,6 +347,11 @@ __device__ inline unsigned laneId() { return __lane_id(); }
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ inline unsigned popc64(uint64_t m) { return unsigned(__popcll(m)); }
 __device__ inline uint64_t lanemaskLt() { return (uint64_t(1) << laneId()) - 1; }
+// number of set bits of m below this lane (v_mbcnt)
+__device__ inline uint32_t rankIn(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
 
 template<class T>
 __device__ inline T waveMin(T v)

@@ -254,12 +254,6 @@ constexpr uint32_t NS_QCAP = 128; // per-quarter queue capacity (ring buffer)
 // v_writelane_b32: `old` with lane `lane` replaced by the wave-uniform value v
 __device__ int writeLane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
-// number of set bits of m below this lane (v_mbcnt)
-__device__ inline uint32_t rankIn(uint64_t m)
-{
-    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-}
-
 template<class Th, bool Sph>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void nsCandidates(GroupsDev G, const uint32_t* __restrict__ groupList,
                                                    const double* __restrict__ x, const double* __restrict__ y,

@@ -27,16 +27,20 @@
 namespace sphx
 {
 
-constexpr int NS_FCAP = 2048; // BFS frontier capacity (per buffer)
-constexpr int NS_LCAP = 1024; // candidate leaves per group
+// traversal capacities: every group is first walked with the small ones (LDS for more
+// waves per CU); the groups that exceed them are walked again with the large ones
+constexpr int NS_FCAP_SMALL = 512;  // BFS frontier capacity (per buffer)
+constexpr int NS_LCAP_SMALL = 512;  // candidate leaves per group
+constexpr int NS_FCAP       = 2048;
+constexpr int NS_LCAP       = 1024; // also the stride of the per-group range lists
 
 struct NsFlags
 {
-    uint32_t           redoCount; // groups appended to the redo list
-    uint32_t           overflow;  // traversal capacity exceeded
-    uint32_t           badGroup;  // a group with more than 64 particles
-    uint32_t           pad;
-    unsigned long long tests;     // exact (target, candidate) distance tests of this pass
+    uint32_t           redoCount;  // groups appended to the redo list
+    uint32_t           overflow;   // traversal capacity exceeded
+    uint32_t           badGroup;   // a group with more than 64 particles
+    uint32_t           retryCount; // groups that exceeded the small traversal capacities
+    unsigned long long tests;      // exact (target, candidate) distance tests of this pass
 };
 
 __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
@@ -53,23 +57,22 @@ __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const
     return ov;
 }
 
-template<class Th>
-__global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t* __restrict__ groupList,
-                                                    const double* __restrict__ x, const double* __restrict__ y,
-                                                    const double* __restrict__ z, const Th* __restrict__ h,
-                                                    const int32_t* __restrict__ childOffsets,
-                                                    const int32_t* __restrict__ internalToLeaf,
-                                                    const uint32_t* __restrict__ layout,
-                                                    const double* __restrict__ centers,
-                                                    const double* __restrict__ sizes, DBox box,
-                                                    uint2* __restrict__ ranges, uint32_t* __restrict__ rangeCount,
-                                                    NsFlags* __restrict__ flags)
+// The ranges of one group g (all 64 lanes of the wave call this).  A group whose walk
+// exceeds the capacities gets no ranges and is appended to retryList, or, without a
+// retry list, raises the overflow flag.
+template<class Th, int FCAP, int LCAP>
+__device__ void groupRangesOf(uint32_t g, GroupsDev G, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, const Th* __restrict__ h,
+                              const int32_t* __restrict__ childOffsets, const int32_t* __restrict__ internalToLeaf,
+                              const uint32_t* __restrict__ layout, const double* __restrict__ centers,
+                              const double* __restrict__ sizes, const DBox& box, uint2* __restrict__ ranges,
+                              uint32_t* __restrict__ rangeCount, uint32_t* __restrict__ retryList,
+                              NsFlags* __restrict__ flags)
 {
-    __shared__ int32_t front[2][NS_FCAP];
-    __shared__ int32_t leafList[NS_LCAP];
+    static_assert((LCAP & (LCAP - 1)) == 0 && LCAP <= NS_LCAP, "leaf capacity: a power of two within the range stride");
+    __shared__ int32_t front[2][FCAP];
+    __shared__ int32_t leafList[LCAP];
 
-    const uint32_t blk  = xcdContiguous(blockIdx.x, gridDim.x);
-    const uint32_t g    = groupList ? groupList[blk] : blk;
     const unsigned lane = threadIdx.x;
     uint32_t       s, e;
     groupRange(G, g, s, e);
@@ -143,12 +146,12 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
 
             uint64_t mL   = ballot(leaf);
             int      posL = lcount + int(popc64(mL & lt));
-            if (leaf && posL < NS_LCAP) leafList[posL] = internalToLeaf[node];
+            if (leaf && posL < LCAP) leafList[posL] = internalToLeaf[node];
             lcount += int(popc64(mL));
 
             uint64_t mI   = ballot(inner);
             int      posI = ncount + 8 * int(popc64(mI & lt));
-            if (inner && posI + 8 <= NS_FCAP)
+            if (inner && posI + 8 <= FCAP)
             {
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
             ncount += 8 * int(popc64(mI));
         }
         __syncthreads();
-        if (lcount > NS_LCAP || ncount > NS_FCAP) overflow = true;
+        if (lcount > LCAP || ncount > FCAP) overflow = true;
         cur    = nxt;
         fcount = ncount;
     }
@@ -165,8 +168,9 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
     {
         if (lane == 0)
         {
-            atomicOr(&flags->overflow, 1u);
             rangeCount[g] = 0;
+            if (retryList) retryList[atomicAdd(&flags->retryCount, 1u)] = g;
+            else atomicOr(&flags->overflow, 1u);
         }
         return;
     }
@@ -221,6 +225,49 @@ __global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t*
         rcount += int(popc64(mS));
     }
     if (lane == 0) rangeCount[g] = uint32_t(rcount);
+}
+
+// all groups (or those of groupList) with the small capacities
+template<class Th>
+__global__ __launch_bounds__(64) void nsGroupRanges(GroupsDev G, const uint32_t* __restrict__ groupList,
+                                                    const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const Th* __restrict__ h,
+                                                    const int32_t* __restrict__ childOffsets,
+                                                    const int32_t* __restrict__ internalToLeaf,
+                                                    const uint32_t* __restrict__ layout,
+                                                    const double* __restrict__ centers,
+                                                    const double* __restrict__ sizes, DBox box,
+                                                    uint2* __restrict__ ranges, uint32_t* __restrict__ rangeCount,
+                                                    uint32_t* __restrict__ retryList, NsFlags* __restrict__ flags)
+{
+    const uint32_t blk = xcdContiguous(blockIdx.x, gridDim.x);
+    const uint32_t g   = groupList ? groupList[blk] : blk;
+    groupRangesOf<Th, NS_FCAP_SMALL, NS_LCAP_SMALL>(g, G, x, y, z, h, childOffsets, internalToLeaf, layout, centers,
+                                                    sizes, box, ranges, rangeCount, retryList, flags);
+}
+
+// the groups of the retry list with the large capacities (grid-stride over the list,
+// whose length the first walk left in flags->retryCount)
+template<class Th>
+__global__ __launch_bounds__(64) void nsGroupRangesRetry(GroupsDev G, const double* __restrict__ x,
+                                                         const double* __restrict__ y, const double* __restrict__ z,
+                                                         const Th* __restrict__ h,
+                                                         const int32_t* __restrict__ childOffsets,
+                                                         const int32_t* __restrict__ internalToLeaf,
+                                                         const uint32_t* __restrict__ layout,
+                                                         const double* __restrict__ centers,
+                                                         const double* __restrict__ sizes, DBox box,
+                                                         uint2* __restrict__ ranges, uint32_t* __restrict__ rangeCount,
+                                                         const uint32_t* __restrict__ retryList,
+                                                         NsFlags* __restrict__ flags)
+{
+    const uint32_t n = flags->retryCount;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x)
+    {
+        groupRangesOf<Th, NS_FCAP, NS_LCAP>(retryList[k], G, x, y, z, h, childOffsets, internalToLeaf, layout,
+                                             centers, sizes, box, ranges, rangeCount, nullptr, flags);
+        __syncthreads(); // the next group reuses the LDS lists
+    }
 }
 
 // updateH (sph/kernels.hpp:27-33)
@@ -585,8 +632,9 @@ __global__ void nsResetFlags(NsFlags* f, int clearErrors)
 {
     if (threadIdx.x == 0)
     {
-        f->redoCount = 0;
-        f->tests     = 0;
+        f->redoCount  = 0;
+        f->retryCount = 0;
+        f->tests      = 0;
         if (clearErrors)
         {
             f->overflow = 0;
@@ -709,8 +757,9 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
     uint32_t* rcnt   = scratchT<uint32_t>(ctx, S_NS_RCOUNT, numG);
     uint8_t*  iter   = scratchT<uint8_t>(ctx, S_NS_STATE, size_t(G.last));
     uint32_t* redo   = scratchT<uint32_t>(ctx, S_NS_REDO, 2 * size_t(numG));
+    uint32_t* retry  = scratchT<uint32_t>(ctx, S_NS_RETRY, numG);
     NsFlags*  flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
-    if (!ranges || !rcnt || !iter || !redo || !flags) return SPHX_ERR_NOMEM;
+    if (!ranges || !rcnt || !iter || !redo || !retry || !flags) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
 
     if (G.last > G.first)
@@ -732,9 +781,17 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
             KTimer kt_(ctx, "nsGroupRanges");
             nsGroupRanges<Th><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, t->childOffsets,
                                                                 t->internalToLeaf, t->layout, t->centers, t->sizes,
-                                                                db, ranges, rcnt, flags);
+                                                                db, ranges, rcnt, retry, flags);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsGroupRanges");
+        {
+            KTimer   kt_(ctx, "nsGroupRangesRetry");
+            uint32_t grid = std::min<uint32_t>(numLaunch, uint32_t(ctx->numCU) * 4u);
+            nsGroupRangesRetry<Th><<<grid, 64, 0, ctx->stream>>>(G, x, y, z, h, t->childOffsets, t->internalToLeaf,
+                                                                 t->layout, t->centers, t->sizes, db, ranges, rcnt,
+                                                                 retry, flags);
+        }
+        SPHX_LAUNCH_CHECK(ctx, "nsGroupRangesRetry");
         {
             KTimer kt_(ctx, "nsCandidates");
             nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, iter, ranges, rcnt, db,

@@ -174,6 +174,51 @@ def test_find_neighbors_bit_exact(ctx, periodic, ng, ngmax, n):
         assert np.array_equal(nb[i, :c], ref_nb[i, :c]), i
 
 
+def _walk_counts(rt, p, r):
+    """Breadth-first walk of the oracle's tree with the search box of points p (radius r,
+    periodic unit box), as the GPU walk does it: (candidate leaves, largest frontier)."""
+    co, C, S = rt["childOffsets"], rt["centers"], rt["sizes"]
+    glo, ghi = (p - r).min(0), (p + r).max(0)
+    gc, gh = 0.5 * (glo + ghi), 0.5 * (ghi - glo)
+    front, leaves, widest = list(range(co[0], co[0] + 8)), 0, 8
+    while front:
+        d = C[front] - gc
+        d -= np.rint(d)
+        ov = np.all(np.abs(d) <= S[front] + gh, axis=1)
+        nxt = []
+        for node, o in zip(front, ov):
+            if o and co[node] == 0:
+                leaves += 1
+            elif o:
+                nxt += list(range(co[node], co[node] + 8))
+        front, widest = nxt, max(widest, len(nxt))
+    return leaves, widest
+
+
+def test_find_neighbors_wide_search_boxes(ctx):
+    """A group whose tree walk exceeds the first walk's capacities (512 frontier nodes or
+    candidate leaves) is walked again with the larger ones; its lists stay bit-exact."""
+    from sphexa_amd import hydro
+
+    print("case find_neighbors wide", flush=True)
+    n, ngmax, bucket = 60000, 256, 32
+    inp, box = uniform_box(n, seed=13, ng=100, periodic=True)
+    keys, s, t = _gpu_tree(ctx, inp, box, bucket=bucket)
+    s["h"][:64] = 0.07  # one fixed group with search spheres of radius 0.14
+    rt = CPU.tree(keys, box, bucket=bucket)
+    p = np.stack([s["x"][:64], s["y"][:64], s["z"][:64]], 1)
+    leaves, widest = _walk_counts(rt, p, 0.14)
+    assert max(leaves, widest) > 512 and leaves <= 1024 and widest <= 2048, (leaves, widest)
+    ref_nb, ref_nc = CPU.find_neighbors(s["x"], s["y"], s["z"], s["h"], box, rt, ngmax)
+    nb, nc = hydro.find_neighbors(ctx, dev(s["x"]), dev(s["y"]), dev(s["z"]), dev(s["h"]), hbox(box), t, ngmax)
+    nc = u32(nc)
+    assert np.array_equal(nc, ref_nc)
+    nb = u32(nb.reshape(-1)).reshape(n, ngmax)
+    for i in list(range(64)) + list(range(64, n, 97)):
+        c = min(nc[i], ngmax)
+        assert np.array_equal(nb[i, :c], ref_nb[i, :c]), i
+
+
 def _run_gpu_step(ctx, inp, box, precision, ng0=100, ngmax=150, exact=False):
     from sphexa_amd import hydro
 

@@ -20,6 +20,7 @@
 // re-runs only the groups that still have particles outside [ng0/4, ngmax + 1].
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <type_traits>
 
 #include "sphx_internal.hpp"
@@ -34,6 +35,11 @@ constexpr int NS_LCAP_SMALL = 512;  // candidate leaves per group
 constexpr int NS_FCAP       = 2048;
 constexpr int NS_LCAP       = 1024; // also the stride of the per-group range lists
 
+// the per-group test counts are added to NS_TEST_SLOTS counters (group g to counter
+// g mod NS_TEST_SLOTS) and summed after the pass: a single counter would serialize one
+// atomic per group
+constexpr uint32_t NS_TEST_SLOTS = 256;
+
 struct NsFlags
 {
     uint32_t           redoCount;  // groups appended to the redo list
@@ -41,6 +47,7 @@ struct NsFlags
     uint32_t           badGroup;   // a group with more than 64 particles
     uint32_t           retryCount; // groups that exceeded the small traversal capacities
     unsigned long long tests;      // exact (target, candidate) distance tests of this pass
+    unsigned long long testSlots[NS_TEST_SLOTS];
 };
 
 __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
@@ -586,7 +593,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
         if (done) break;
     }
     // exact pair tests of this pass
-    if (lane == 0 && tests) atomicAdd(&flags->tests, tests);
+    if (lane == 0 && tests) atomicAdd(&flags->testSlots[g % NS_TEST_SLOTS], tests);
 
     if constexpr (Sph)
     {
@@ -628,8 +635,9 @@ __global__ __launch_bounds__(256) void nsInitState(uint8_t* iter, uint32_t first
     iter[i] = 0;
 }
 
-__global__ void nsResetFlags(NsFlags* f, int clearErrors)
+__global__ __launch_bounds__(NS_TEST_SLOTS) void nsResetFlags(NsFlags* f, int clearErrors)
 {
+    f->testSlots[threadIdx.x] = 0;
     if (threadIdx.x == 0)
     {
         f->redoCount  = 0;
@@ -640,6 +648,21 @@ __global__ void nsResetFlags(NsFlags* f, int clearErrors)
             f->overflow = 0;
             f->badGroup = 0;
         }
+    }
+}
+
+__global__ __launch_bounds__(NS_TEST_SLOTS) void nsSumTests(NsFlags* f)
+{
+    __shared__ unsigned long long part[NS_TEST_SLOTS / 64];
+    unsigned long long            v = waveSum(f->testSlots[threadIdx.x]);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long tot = 0;
+        for (uint32_t k = 0; k < NS_TEST_SLOTS / 64; ++k)
+            tot += part[k];
+        f->tests = tot;
     }
 }
 
@@ -767,7 +790,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         nsInitState<<<gridFor(G.last - G.first, 256), 256, 0, ctx->stream>>>(iter, G.first, G.last);
         SPHX_LAUNCH_CHECK(ctx, "nsInitState");
     }
-    nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 1);
+    nsResetFlags<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags, 1);
     SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
     ctx->nsTests = 0;
 
@@ -798,8 +821,10 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
                                                                     ngmax, ng0, nbrs, lists, nc, outList, flags);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
-        NsFlags hf;
-        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));
+        nsSumTests<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags);
+        SPHX_LAUNCH_CHECK(ctx, "nsSumTests");
+        NsFlags hf; // the counters before testSlots
+        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, offsetof(NsFlags, testSlots), hipMemcpyDeviceToHost, ctx->stream));
         SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->nsTests += double(hf.tests);
         if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
@@ -808,7 +833,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         numLaunch = hf.redoCount;
         list      = outList;
         pingpong ^= 1;
-        nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 0);
+        nsResetFlags<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags, 0);
         SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
     }
     return SPHX_OK;

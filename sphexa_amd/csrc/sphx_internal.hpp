@@ -38,6 +38,7 @@ enum Slot : int
     S_NS_STATE,      // per-particle h-iteration counters / active flags
     S_NS_REDO,       // groups that need another h iteration
     S_NS_RETRY,      // groups whose tree walk exceeded the small capacities
+    S_NS_TESTS,      // candidate tests per launched group
     S_GROUPS,        // computed group boundaries
     S_GROUPS_TMP,    // group split scratch
     S_SMALL,         // small device scalars (flags, counters, reductions)

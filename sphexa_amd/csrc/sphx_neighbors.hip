@@ -35,11 +35,6 @@ constexpr int NS_LCAP_SMALL = 512;  // candidate leaves per group
 constexpr int NS_FCAP       = 2048;
 constexpr int NS_LCAP       = 1024; // also the stride of the per-group range lists
 
-// the per-group test counts are added to NS_TEST_SLOTS counters (group g to counter
-// g mod NS_TEST_SLOTS) and summed after the pass: a single counter would serialize one
-// atomic per group
-constexpr uint32_t NS_TEST_SLOTS = 256;
-
 struct NsFlags
 {
     uint32_t           redoCount;  // groups appended to the redo list
@@ -47,7 +42,6 @@ struct NsFlags
     uint32_t           badGroup;   // a group with more than 64 particles
     uint32_t           retryCount; // groups that exceeded the small traversal capacities
     unsigned long long tests;      // exact (target, candidate) distance tests of this pass
-    unsigned long long testSlots[NS_TEST_SLOTS];
 };
 
 __device__ inline bool nodeOverlapsBox(const double* __restrict__ centers, const double* __restrict__ sizes, int node,
@@ -316,7 +310,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                                                    const uint32_t* __restrict__ rangeCount, DBox box, uint32_t ngmax,
                                                    uint32_t ng0, uint32_t* __restrict__ nbrs, SphLists out,
                                                    uint32_t* __restrict__ nc, uint32_t* __restrict__ redoList,
-                                                   NsFlags* __restrict__ flags)
+                                                   NsFlags* __restrict__ flags,
+                                                   unsigned long long* __restrict__ groupTests)
 {
 #pragma clang fp contract(off)
     __shared__ uint32_t qj[4][NS_QCAP]; // queued candidate indices per quarter
@@ -593,7 +588,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
         if (done) break;
     }
     // exact pair tests of this pass
-    if (lane == 0 && tests) atomicAdd(&flags->testSlots[g % NS_TEST_SLOTS], tests);
+    // (stored per launched group and summed by nsSumTests: one atomic per group on a
+    // single counter would serialize)
+    if (lane == 0) groupTests[blk] = tests;
 
     if constexpr (Sph)
     {
@@ -635,9 +632,8 @@ __global__ __launch_bounds__(256) void nsInitState(uint8_t* iter, uint32_t first
     iter[i] = 0;
 }
 
-__global__ __launch_bounds__(NS_TEST_SLOTS) void nsResetFlags(NsFlags* f, int clearErrors)
+__global__ void nsResetFlags(NsFlags* f, int clearErrors)
 {
-    f->testSlots[threadIdx.x] = 0;
     if (threadIdx.x == 0)
     {
         f->redoCount  = 0;
@@ -651,18 +647,21 @@ __global__ __launch_bounds__(NS_TEST_SLOTS) void nsResetFlags(NsFlags* f, int cl
     }
 }
 
-__global__ __launch_bounds__(NS_TEST_SLOTS) void nsSumTests(NsFlags* f)
+// flags->tests += sum of the per-group test counts (one atomic per block)
+__global__ __launch_bounds__(256) void nsSumTests(const unsigned long long* __restrict__ groupTests, uint32_t n,
+                                                  NsFlags* __restrict__ f)
 {
-    __shared__ unsigned long long part[NS_TEST_SLOTS / 64];
-    unsigned long long            v = waveSum(f->testSlots[threadIdx.x]);
+    __shared__ unsigned long long part[4];
+    unsigned long long            v = 0;
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u)
+        v += groupTests[k];
+    v = waveSum(v);
     if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        unsigned long long tot = 0;
-        for (uint32_t k = 0; k < NS_TEST_SLOTS / 64; ++k)
-            tot += part[k];
-        f->tests = tot;
+        unsigned long long tot = part[0] + part[1] + part[2] + part[3];
+        if (tot) atomicAdd(&f->tests, tot);
     }
 }
 
@@ -781,8 +780,9 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
     uint8_t*  iter   = scratchT<uint8_t>(ctx, S_NS_STATE, size_t(G.last));
     uint32_t* redo   = scratchT<uint32_t>(ctx, S_NS_REDO, 2 * size_t(numG));
     uint32_t* retry  = scratchT<uint32_t>(ctx, S_NS_RETRY, numG);
+    auto*     gtests = scratchT<unsigned long long>(ctx, S_NS_TESTS, numG);
     NsFlags*  flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
-    if (!ranges || !rcnt || !iter || !redo || !retry || !flags) return SPHX_ERR_NOMEM;
+    if (!ranges || !rcnt || !iter || !redo || !retry || !gtests || !flags) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
 
     if (G.last > G.first)
@@ -790,7 +790,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         nsInitState<<<gridFor(G.last - G.first, 256), 256, 0, ctx->stream>>>(iter, G.first, G.last);
         SPHX_LAUNCH_CHECK(ctx, "nsInitState");
     }
-    nsResetFlags<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags, 1);
+    nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 1);
     SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
     ctx->nsTests = 0;
 
@@ -818,13 +818,15 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         {
             KTimer kt_(ctx, "nsCandidates");
             nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, iter, ranges, rcnt, db,
-                                                                    ngmax, ng0, nbrs, lists, nc, outList, flags);
+                                                                    ngmax, ng0, nbrs, lists, nc, outList, flags,
+                                                                    gtests);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
-        nsSumTests<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags);
+        nsSumTests<<<std::min<uint32_t>(256u, (numLaunch + 1023u) / 1024u), 256, 0, ctx->stream>>>(gtests, numLaunch,
+                                                                                                flags);
         SPHX_LAUNCH_CHECK(ctx, "nsSumTests");
-        NsFlags hf; // the counters before testSlots
-        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, offsetof(NsFlags, testSlots), hipMemcpyDeviceToHost, ctx->stream));
+        NsFlags hf;
+        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));
         SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->nsTests += double(hf.tests);
         if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
@@ -833,7 +835,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         numLaunch = hf.redoCount;
         list      = outList;
         pingpong ^= 1;
-        nsResetFlags<<<1, NS_TEST_SLOTS, 0, ctx->stream>>>(flags, 0);
+        nsResetFlags<<<1, 64, 0, ctx->stream>>>(flags, 0);
         SPHX_LAUNCH_CHECK(ctx, "nsResetFlags");
     }
     return SPHX_OK;

@@ -102,7 +102,7 @@ typedef struct sphx_particles
     uint32_t      ng0, ngmax;
     float         Kcour;
     double        K, gamma;
-    const double *x, *y, *z;
+    double *      x, *y, *z; /* written only by sphx_compute_positions                  */
     void *        h, *m, *vx, *vy, *vz, *u;
     void *        rho, *p, *c;
     void *        c11, *c12, *c13, *c22, *c23, *c33;
@@ -110,6 +110,9 @@ typedef struct sphx_particles
     uint32_t*     nc;
     const void*   wh;
     double        minDtCourant; /* output of sphx_compute_momentum_energy */
+    /* integration state (particles_data.hpp:219,236), used by sphx_compute_positions only:
+     * x_m1, y_m1, z_m1 = X_n - X_{n-1}, du_m1 = du of the previous step; field type */
+    void *x_m1, *y_m1, *z_m1, *du_m1;
 } sphx_particles;
 
 #define SPHX_TABLE_SIZE 20000 /* sph::lt::kTableSize, sph/include/sph/table_lookup.hpp:10 */
@@ -271,6 +274,43 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
  * the stream once to return minDtCourant. */
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                  const sphx_box* box);
+
+/* ---- integration step ------------------------------------------------------ */
+/* The dataset's global time-step scalars (sph/include/sph/particles_data.hpp:93-111,138). */
+typedef struct sphx_timestep
+{
+    double minDt, minDt_m1, ttot; /* state: updated by sphx_compute_timestep               */
+    double minDtCourant;          /* candidate limits: Courant (sphx_compute_momentum_energy */
+    double minDtRho;              /* output) and any other limit (INFINITY when unused)     */
+    double g, etaAcc;             /* gravitational constant; the acceleration limit is used  */
+                                  /* when g != 0 (accelerationTimestep, ts_global.hpp:48-71) */
+    double minDtAcc;              /* output: that limit (INFINITY when g == 0)               */
+} sphx_timestep;
+
+/* sph::computeTimestep (sph/include/sph/ts_global.hpp:100-130) on one rank:
+ * minDt_new = min(minDtAcc, minDtCourant, minDtRho, 1.1 minDt), where minDtAcc =
+ * etaAcc * (min_i h_i^2 / |a_i|^2)^(1/4) over [first, last) when g != 0; then
+ * ttot += minDt_new, minDt_m1 = minDt, minDt = minDt_new.  Synchronises the stream
+ * once when g != 0 (the minimum is returned on the host). */
+int sphx_compute_timestep(sphx_ctx* ctx, uint32_t first, uint32_t last, const sphx_particles* d,
+                          sphx_timestep* ts);
+
+/* sph::computePositions, CPU semantics (sph/include/sph/positions.hpp:113-137,162-179,
+ * 207-230): for the particles of `groups`, the 2nd-order Press position update
+ * positionUpdate(minDt, minDt_m1, X, A, X_m1, box) (:100-111, with putInBox for periodic
+ * dimensions, cstone/sfc/box.hpp:193-214) writes x, y, z, x_m1, y_m1, z_m1 and vx, vy, vz; near fixed
+ * boundaries A and X_m1 are scaled by fbcAdjustFactors (:47-74).  Then the Adams-Bashforth
+ * energy update u = energyUpdate(u, minDt f, minDt_m1 f, du, du_m1) (:77-84), f the smallest
+ * factor, and du_m1 = du.  Uses ts->minDt and ts->minDt_m1 as sphx_compute_timestep left them. */
+int sphx_compute_positions(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box,
+                           const sphx_timestep* ts);
+
+/* sph::updateSmoothingLength (sph/include/sph/update_h.hpp:13-49): for the particles of
+ * `groups`, h = updateH(ng0, nc, h) (kernels.hpp:27-33); particles with nc <= 1 get
+ * keys[i] = removeKey (2^63; keys may be NULL) and set *keysRemoved = 1 (else 0).
+ * Synchronises the stream once (to return *keysRemoved). */
+int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, uint64_t* keys,
+                                 int* keysRemoved);
 
 /* ---- adjoints -------------------------------------------------------------- */
 /* No counterpart in the reference (its loops are forward-only).  Vector-Jacobian

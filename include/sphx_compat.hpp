@@ -282,4 +282,48 @@ void computeMomentumEnergySTD(const GroupView& groups, Dataset& d, const Box& bo
     d.minDtCourant = p.minDtCourant;
 }
 
+// ---- sph::computeTimestep (sph/include/sph/ts_global.hpp:100-130) on one rank: updates
+// d.minDt, d.minDt_m1 and d.ttot from d.minDtCourant, d.minDtRho, 1.1 d.minDt and, when
+// d.g != 0, the acceleration limit (extra limits: fold them into d.minDtRho first)
+template<class Dataset>
+void computeTimestep(size_t first, size_t last, Dataset& d)
+{
+    sphx_particles p = toParticles(d);
+    sphx_timestep  ts{};
+    ts.minDt = double(d.minDt), ts.minDt_m1 = double(d.minDt_m1), ts.ttot = double(d.ttot);
+    ts.minDtCourant = double(d.minDtCourant), ts.minDtRho = double(d.minDtRho);
+    ts.g = double(d.g), ts.etaAcc = double(d.etaAcc);
+    check(sphx_compute_timestep(context(), uint32_t(first), uint32_t(last), &p, &ts));
+    d.minDt = ts.minDt, d.minDt_m1 = ts.minDt_m1, d.ttot = ts.ttot;
+}
+
+// ---- sph::computePositions (sph/include/sph/positions.hpp:207-230) with the CPU branch's
+// semantics: the step is d.minDt, d.minDt_m1 (dt_forward and dt_m1 are the GPU branch's
+// float copies of them); block time steps (rung != nullptr) are not supported
+template<class GroupView, class Dataset, class Box, class DtM1>
+void computePositions(const GroupView& groups, Dataset& d, const Box& box, float /*dt_forward*/,
+                      const DtM1& /*dt_m1*/, const uint8_t* rung = nullptr)
+{
+    if (rung) throw std::runtime_error("sphx_compat::computePositions: rungs are not supported");
+    sphx_particles p = toParticles(d);
+    p.x_m1 = rawPtr(d.x_m1), p.y_m1 = rawPtr(d.y_m1), p.z_m1 = rawPtr(d.z_m1), p.du_m1 = rawPtr(d.du_m1);
+    sphx_groups   g = toGroups(groups);
+    sphx_box      b = toBox(box);
+    sphx_timestep ts{};
+    ts.minDt = double(d.minDt), ts.minDt_m1 = double(d.minDt_m1);
+    check(sphx_compute_positions(context(), &g, &p, &b, &ts));
+}
+
+// ---- sph::updateSmoothingLength (sph/include/sph/update_h.hpp:35-49): returns whether a
+// particle was left with nc <= 1 (its key becomes removeKey)
+template<class GroupView, class Dataset>
+bool updateSmoothingLength(const GroupView& groups, Dataset& d)
+{
+    sphx_particles p = toParticles(d);
+    sphx_groups    g = toGroups(groups);
+    int            removed = 0;
+    check(sphx_update_smoothing_length(context(), &g, &p, reinterpret_cast<uint64_t*>(rawPtr(d.keys)), &removed));
+    return removed != 0;
+}
+
 } // namespace sphx_compat

@@ -113,6 +113,13 @@ class Cpu:
             f("momentum_energy", _D, [_U64, _U, _U, _V, _V, _U] + [_V] * 18 + [_D, C.c_float, _V] + [_V] * 4)
             f("hydro_step", _D, [_U64] + [_V] * 9 + [_V, _U, _U, _U, _D, _D, C.c_float] + [_V] * 5)
         f("xmass_one", _D, [_U, _V, _U, _V, _V, _V, _V, _V, _V, _D, _V])
+        if kind == "oracle":
+            f("min_h2_a2", _D, [_U, _U, _V, _V, _V, _V])
+            f("compute_timestep", _D, [_U, _U, _V, _V, _V, _V, _D, _D, _D, _D, _V])
+            f("compute_positions", None, [_U, _U] + [_V] * 16 + [_D, _D, _V])
+        else:
+            f("compute_positions", None, [_U64, _U, _U] + [_V] * 16 + [_D, _D, _V])
+        f("update_smoothing_length", _I, [_U, _U, _U, _V, _V, _V])
 
     def _f(self, name, res, args):
         fn = getattr(self.lib, f"{self.pre}_{name}")
@@ -289,6 +296,43 @@ class Cpu:
     def xmass_one(self, i, nbrs, x, y, z, h, m, wh, K, box):
         nb = np.ascontiguousarray(nbrs, np.uint32)
         return self._xmass_one(i, _p(nb), len(nb), _p(x), _p(y), _p(z), _p(h), _p(m), _p(wh), K, C.byref(box))
+
+
+    # ---------------------------------------------------------------- integration step
+    def compute_timestep(self, ts, h=None, ax=None, ay=None, az=None, g=0.0, eta_acc=0.2, min_dt_courant=np.inf,
+                         min_dt_rho=np.inf, first=0, last=None):
+        """computeTimestep on one rank; ts = [minDt, minDt_m1, ttot] (updated copy returned)
+        with minDtAcc."""
+        t = np.array(ts, dtype=np.float64)
+        n = 0 if h is None else len(h)
+        last = n if last is None else last
+        acc = self._compute_timestep(first, last, _p(h), _p(ax), _p(ay), _p(az), g, eta_acc, min_dt_courant,
+                                     min_dt_rho, _p(t))
+        return t, acc
+
+    def min_h2_a2(self, h, ax, ay, az):
+        return self._min_h2_a2(0, len(h), _p(h), _p(ax), _p(ay), _p(az))
+
+    def compute_positions(self, st, dt, dt_m1, box, first=0, last=None):
+        """updatePositionsHost + updateIntEnergyHost on copies of the state dict st (keys x, y,
+        z, x_m1, y_m1, z_m1, vx, vy, vz, ax, ay, az, h, u, du, du_m1, all float64); returns
+        the updated copy."""
+        o = {k: np.array(v, dtype=np.float64) for k, v in st.items()}
+        n = len(o["x"])
+        last = n if last is None else last
+        args = [_p(o[k]) for k in ("x", "y", "z", "x_m1", "y_m1", "z_m1", "vx", "vy", "vz", "ax", "ay", "az", "h",
+                                   "u", "du", "du_m1")]
+        self._compute_positions(*self._n(n), first, last, *args, dt, dt_m1, C.byref(box))
+        return o
+
+    def update_smoothing_length(self, ng0, nc, h, keys=None, first=0, last=None):
+        """updateSmoothingLengthCpu on copies: (h, keys or None, keysRemoved)."""
+        h = np.array(h, dtype=np.float64)
+        nc = np.ascontiguousarray(nc, np.uint32)
+        k = None if keys is None else np.array(keys, dtype=np.uint64)
+        last = len(h) if last is None else last
+        r = self._update_smoothing_length(first, last, ng0, _p(nc), _p(h), _p(k))
+        return h, k, bool(r)
 
 
 def available(kind):

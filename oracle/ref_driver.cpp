@@ -33,7 +33,9 @@
 #include "sph/hydro_std/eos.hpp"
 #include "sph/hydro_std/iad.hpp"
 #include "sph/hydro_std/momentum_energy.hpp"
+#include "sph/positions.hpp"
 #include "sph/sph_kernel_tables.hpp"
+#include "sph/update_h.hpp"
 
 #include "../include/sphx.h"
 
@@ -80,6 +82,14 @@ struct MiniData
     double K{0}, gamma{5.0 / 3.0}, muiConst{10.0}, minDtCourant{INFINITY};
     double Kcour{0.2};
     unsigned ngmax{150}, ng0{100};
+};
+
+// the subset of sphexa::ParticlesData that updatePositionsHost / updateIntEnergyHost read
+struct IntData
+{
+    using RealType = double;
+    Field<double> x, y, z, x_m1, y_m1, z_m1, vx, vy, vz, ax, ay, az, h, u, du, du_m1;
+    double        minDt{0}, minDt_m1{0};
 };
 
 template<class T>
@@ -362,6 +372,31 @@ double ref_hydro_step(uint64_t n, const double* x0, const double* y0, const doub
     stage_s[9] = 0;
     stage_s[10] = 0;
     return d.minDtCourant;
+}
+
+
+// sph::updatePositionsHost + updateIntEnergyHost (sph/include/sph/positions.hpp:113-179),
+// the CPU branch of computePositions (:207-230)
+void ref_compute_positions(uint64_t n, uint32_t first, uint32_t last, double* x, double* y, double* z, double* x_m1,
+                           double* y_m1, double* z_m1, double* vx, double* vy, double* vz, const double* ax,
+                           const double* ay, const double* az, const double* h, double* u, const double* du,
+                           double* du_m1, double dt, double dt_m1, const sphx_box* b)
+{
+    IntData d;
+    d.x = F(x, n), d.y = F(y, n), d.z = F(z, n), d.x_m1 = F(x_m1, n), d.y_m1 = F(y_m1, n), d.z_m1 = F(z_m1, n);
+    d.vx = F(vx, n), d.vy = F(vy, n), d.vz = F(vz, n), d.ax = F(ax, n), d.ay = F(ay, n), d.az = F(az, n);
+    d.h = F(h, n), d.u = F(u, n), d.du = F(du, n), d.du_m1 = F(du_m1, n);
+    d.minDt = dt, d.minDt_m1 = dt_m1;
+    auto box = toBox(b);
+    sph::updatePositionsHost(first, last, d, box);
+    sph::updateIntEnergyHost(first, last, d, box);
+}
+
+// sph::updateSmoothingLengthCpu (sph/include/sph/update_h.hpp:13-32)
+int ref_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, const unsigned* nc, double* h,
+                                uint64_t* keys)
+{
+    return sph::updateSmoothingLengthCpu(first, last, ng0, nc, h, keys) ? 1 : 0;
 }
 
 } // extern "C"

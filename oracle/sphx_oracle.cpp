@@ -28,6 +28,9 @@
  *   tables, K   Simpson known answers (sph/test/table_creation.cpp:17-40),
  *               closed-form integral.
  *   EOS         closed form.
+ *   integration positionUpdate time reversal and constant-acceleration kinematics,
+ *               energyUpdate closed forms, putInBox folds, the timestep minimum
+ *               rule, and equality with oracle/_ref (positions, energy, h update).
  */
 #include <algorithm>
 #include <cmath>
@@ -1028,6 +1031,127 @@ void orc_make_global_box(const double* x, const double* y, const double* z, uint
         }
         *lo[d] = mn, *hi[d] = mx;
     }
+}
+
+
+// --------------------------------------------------------------------------- integration
+// accelerationTimestep, CPU branch (sph/include/sph/ts_global.hpp:48-71) without the
+// final etaAcc * pow(., 0.25): min over [first, last) of h_i^2 / |a_i|^2; a NaN term
+// leaves the minimum unchanged (std::min(m, NaN) returns m); INFINITY if empty
+double orc_min_h2_a2(uint32_t first, uint32_t last, const double* h, const double* ax, const double* ay,
+                     const double* az)
+{
+    double m = INFINITY;
+    for (uint32_t i = first; i < last; ++i)
+    {
+        double a2 = ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]; // norm2, util/array.hpp
+        m         = std::min(m, h[i] * h[i] / a2);
+    }
+    return m;
+}
+
+// computeTimestep (ts_global.hpp:100-130) on one rank (the MPI_Allreduce of one value
+// is the identity); ts = {minDt, minDt_m1, ttot} updated in place; returns minDtAcc
+double orc_compute_timestep(uint32_t first, uint32_t last, const double* h, const double* ax, const double* ay,
+                            const double* az, double g, double etaAcc, double minDtCourant, double minDtRho,
+                            double* ts)
+{
+    const double maxDtIncrease = 1.1; // particles_data.hpp:138
+    double minDtAcc = INFINITY;
+    if (g != 0.0)
+    {
+        minDtAcc = last > first ? etaAcc * std::pow(orc_min_h2_a2(first, last, h, ax, ay, az), 0.25) : INFINITY;
+    }
+    double minDt = std::min({minDtAcc, minDtCourant, minDtRho, maxDtIncrease * ts[0]});
+    ts[2] += minDt;
+    ts[1] = ts[0];
+    ts[0] = minDt;
+    return minDtAcc;
+}
+
+// fbcAdjustFactors (sph/include/sph/positions.hpp:47-74)
+static void fbcFactors(const double X[3], const Box& box, double hi, double f[3])
+{
+    const double width = 2.0, invWidth = 1.0 / width, offset = 1.0;
+    for (int j = 0; j < 3; ++j)
+    {
+        f[j] = 1.0;
+        double md = std::min(std::abs(box.hi[j] - X[j]), std::abs(box.lo[j] - X[j])) * (1.0 / hi);
+        if (box.bc[j] == SPHX_BC_FIXED && md < 2 * width + offset)
+        {
+            f[j] = 0.5 * (std::tanh(4 * (md - offset) * invWidth - 4) + 1);
+        }
+    }
+}
+
+// energyUpdate (positions.hpp:77-84)
+static double energyUpdate(double u_old, double dt, double dt_m1, double du, double du_m1)
+{
+    double u_new = u_old + du * dt + 0.5 * (du - du_m1) / dt_m1 * std::abs(dt) * dt;
+    if (u_new < 0.) { u_new = u_old * std::exp(u_new * dt / u_old); }
+    return u_new;
+}
+
+// updatePositionsHost (positions.hpp:113-137) with positionUpdate (:100-111) and putInBox
+// (cstone/sfc/box.hpp:193-214), then updateIntEnergyHost (:162-179)
+void orc_compute_positions(uint32_t first, uint32_t last, double* x, double* y, double* z, double* x_m1,
+                           double* y_m1, double* z_m1, double* vx, double* vy, double* vz, const double* ax,
+                           const double* ay, const double* az, const double* h, double* u, const double* du,
+                           double* du_m1, double dt, double dt_m1, const sphx_box* b)
+{
+    Box box = makeBox(b);
+    bool anyFBC = box.bc[0] == SPHX_BC_FIXED || box.bc[1] == SPHX_BC_FIXED || box.bc[2] == SPHX_BC_FIXED;
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = first; i < last; ++i)
+    {
+        double X[3] = {x[i], y[i], z[i]};
+        double f[3] = {1.0, 1.0, 1.0};
+        if (anyFBC) fbcFactors(X, box, h[i], f);
+        double A[3]   = {ax[i] * f[0], ay[i] * f[1], az[i] * f[2]};
+        double dXn[3] = {x_m1[i] * f[0], y_m1[i] * f[1], z_m1[i] * f[2]};
+        double V[3], dX[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            double Vnmhalf = dXn[k] * (1.0 / dt_m1);
+            double Vn      = Vnmhalf + 0.5 * dt_m1 * A[k];
+            V[k]           = Vn + A[k] * dt;
+            dX[k]          = (Vn + 0.5 * A[k] * std::abs(dt)) * dt;
+            X[k]           = X[k] + dX[k];
+            if (box.bc[k] == SPHX_BC_PERIODIC && X[k] > box.hi[k]) { X[k] -= box.len[k]; }
+            else if (box.bc[k] == SPHX_BC_PERIODIC && X[k] < box.lo[k]) { X[k] += box.len[k]; }
+        }
+        x[i] = X[0], y[i] = X[1], z[i] = X[2];
+        x_m1[i] = dX[0], y_m1[i] = dX[1], z_m1[i] = dX[2];
+        vx[i] = V[0], vy[i] = V[1], vz[i] = V[2];
+        // updateIntEnergyHost runs after every position is updated: its factor is that
+        // of the new position (the reference GPU kernel uses the old one)
+        double fe = 1.0;
+        if (anyFBC)
+        {
+            fbcFactors(X, box, h[i], f);
+            fe = std::min({f[0], f[1], f[2]});
+        }
+        u[i]      = energyUpdate(u[i], dt * fe, dt_m1 * fe, du[i], du_m1[i]);
+        du_m1[i]  = du[i];
+    }
+}
+
+// updateSmoothingLengthCpu (sph/include/sph/update_h.hpp:13-32); keys may be NULL;
+// returns keysRemoved
+int orc_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, const unsigned* nc, double* h,
+                                uint64_t* keys)
+{
+    int removed = 0;
+    for (uint32_t i = first; i < last; ++i)
+    {
+        if (nc[i] <= 1)
+        {
+            if (keys) keys[i] = kRemoveKey;
+            removed = 1;
+        }
+        h[i] = orc_update_h(ng0, nc[i], h[i]);
+    }
+    return removed;
 }
 
 } // extern "C"

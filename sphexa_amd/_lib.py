@@ -55,7 +55,12 @@ class Particles(C.Structure):
                 ("gamma", _D), ("x", _V), ("y", _V), ("z", _V), ("h", _V), ("m", _V), ("vx", _V), ("vy", _V),
                 ("vz", _V), ("u", _V), ("rho", _V), ("p", _V), ("c", _V), ("c11", _V), ("c12", _V), ("c13", _V),
                 ("c22", _V), ("c23", _V), ("c33", _V), ("ax", _V), ("ay", _V), ("az", _V), ("du", _V), ("nc", _V),
-                ("wh", _V), ("minDtCourant", _D)]
+                ("wh", _V), ("minDtCourant", _D), ("x_m1", _V), ("y_m1", _V), ("z_m1", _V), ("du_m1", _V)]
+
+
+class Timestep(C.Structure):
+    _fields_ = [("minDt", _D), ("minDt_m1", _D), ("ttot", _D), ("minDtCourant", _D), ("minDtRho", _D), ("g", _D),
+                ("etaAcc", _D), ("minDtAcc", _D)]
 
 
 SIGNATURES = {
@@ -93,6 +98,11 @@ SIGNATURES = {
                                        _V, _V]),
     "sphx_momentum_energy_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V,
                                                _V, _V, _V, _V, _V, _V]),
+    "sphx_compute_timestep": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles), C.POINTER(Timestep)]),
+    "sphx_compute_positions": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box),
+                                         C.POINTER(Timestep)]),
+    "sphx_update_smoothing_length": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), _V,
+                                               C.POINTER(C.c_int)]),
     "sphx_make_global_box": (C.c_int, [_V, _V, _V, _V, _U64, C.POINTER(Box), C.POINTER(Box)]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),

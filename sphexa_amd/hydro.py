@@ -18,7 +18,12 @@ this module                    reference entry point
 ``compute_eos_hydro_std``      sph::computeEOS_HydroStd (hydro_std/eos.hpp:144)
 ``compute_iad``                sph::computeIAD (hydro_std/iad.hpp:74)
 ``compute_momentum_energy_std``sph::computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:96)
+``compute_timestep``           sph::computeTimestep (sph/include/sph/ts_global.hpp:101)
+``compute_positions``          sph::computePositions (sph/include/sph/positions.hpp:208)
+``update_smoothing_length``    sph::updateSmoothingLength (sph/include/sph/update_h.hpp:35)
 ``HydroStep.compute_forces``   HydroProp::computeForces (main/src/propagator/std_hydro.hpp:119)
+``HydroStep.integrate``        HydroProp::integrate (main/src/propagator/std_hydro.hpp:170)
+``Simulation.step``            one time step: sync, computeForces, integrate
 =============================  ==============================================================
 
 Errors raise ``SphxError`` carrying the library's status code; there is no CPU
@@ -35,7 +40,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import Box, Groups, Octree, Particles, SphxError, FP64, MIXED, BC_OPEN, BC_PERIODIC, BC_FIXED
+from ._lib import Box, Groups, Octree, Particles, SphxError, Timestep, FP64, MIXED, BC_OPEN, BC_PERIODIC, BC_FIXED
 
 _hip = None
 
@@ -305,20 +310,27 @@ class ParticlesData:
     keys: torch.Tensor | None = None
     wh: torch.Tensor | None = None
     minDtCourant: float = math.inf
+    x_m1: torch.Tensor | None = None
+    y_m1: torch.Tensor | None = None
+    z_m1: torch.Tensor | None = None
+    du_m1: torch.Tensor | None = None
 
     HYDRO = ("h", "m", "vx", "vy", "vz", "u", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33", "ax",
              "ay", "az", "du")
+    # integration state, allocated by allocate(integration=True)
+    INTEGRATION = ("x_m1", "y_m1", "z_m1", "du_m1")
 
     @property
     def dtype(self):
         return torch.float64 if self.precision == FP64 else torch.float32
 
-    def allocate(self, device):
-        """Allocate every missing field, make the kernel table and K."""
+    def allocate(self, device, integration=False):
+        """Allocate every missing field (and the integration state if asked), make the kernel
+        table and K."""
         for f in ("x", "y", "z"):
             if getattr(self, f) is None:
                 setattr(self, f, torch.zeros(self.n, dtype=torch.float64, device=device))
-        for f in self.HYDRO:
+        for f in self.HYDRO + (self.INTEGRATION if integration else ()):
             if getattr(self, f) is None:
                 setattr(self, f, torch.zeros(self.n, dtype=self.dtype, device=device))
         if self.nc is None:
@@ -333,7 +345,7 @@ class ParticlesData:
         s = Particles()
         s.n, s.precision, s.ng0, s.ngmax, s.Kcour, s.K, s.gamma = (self.n, self.precision, self.ng0, self.ngmax,
                                                                    self.Kcour, self.K, self.gamma)
-        for f in ("x", "y", "z", "nc", "wh") + self.HYDRO:
+        for f in ("x", "y", "z", "nc", "wh") + self.HYDRO + self.INTEGRATION:
             t = getattr(self, f)
             setattr(s, f, t.data_ptr() if t is not None else None)
         s.minDtCourant = self.minDtCourant
@@ -419,6 +431,54 @@ def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups
     return d.minDtCourant
 
 
+# ------------------------------------------------------------------------------------------- integration
+@dataclass
+class TimestepState:
+    """The dataset's global time-step scalars (particles_data.hpp:93-111): minDt, minDt_m1 and
+    ttot evolve; g != 0 adds the acceleration limit etaAcc (min h^2/|a|^2)^(1/4)."""
+
+    minDt: float = 1e-12
+    minDt_m1: float = 1e-12
+    ttot: float = 0.0
+    g: float = 0.0
+    etaAcc: float = 0.2
+    minDtRho: float = math.inf
+    minDtAcc: float = math.inf
+
+    def struct(self, min_dt_courant: float) -> Timestep:
+        return Timestep(self.minDt, self.minDt_m1, self.ttot, min_dt_courant, self.minDtRho, self.g, self.etaAcc,
+                        self.minDtAcc)
+
+
+def compute_timestep(ctx: Context, d: ParticlesData, ts: TimestepState, first=0, last=None):
+    """sph::computeTimestep on one rank: ts.minDt becomes min(acceleration limit if g != 0,
+    d.minDtCourant, ts.minDtRho, 1.1 ts.minDt); ts.minDt_m1 and ts.ttot advance."""
+    s = d.struct()
+    t = ts.struct(d.minDtCourant)
+    ctx.check(ctx.lib.sphx_compute_timestep(ctx.h, first, d.n if last is None else last, C.byref(s), C.byref(t)))
+    ts.minDt, ts.minDt_m1, ts.ttot, ts.minDtAcc = t.minDt, t.minDt_m1, t.ttot, t.minDtAcc
+    return ts.minDt
+
+
+def compute_positions(ctx: Context, d: ParticlesData, box: Box, ts: TimestepState, groups=None):
+    """sph::computePositions (CPU semantics): Press position update, Adams-Bashforth energy
+    update, du_m1 = du, with ts.minDt and ts.minDt_m1."""
+    s = d.struct()
+    t = ts.struct(d.minDtCourant)
+    ctx.check(ctx.lib.sphx_compute_positions(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box),
+                                             C.byref(t)))
+
+
+def update_smoothing_length(ctx: Context, d: ParticlesData, keys=None, groups=None) -> bool:
+    """sph::updateSmoothingLength: h from nc; particles with nc <= 1 get removeKey in keys.
+    Returns whether any key was removed."""
+    s = d.struct()
+    r = C.c_int()
+    ctx.check(ctx.lib.sphx_update_smoothing_length(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
+                                                   _ptr(keys), C.byref(r)))
+    return bool(r.value)
+
+
 # ------------------------------------------------------------------------------------------- adjoints
 def density_adjoint(ctx: Context, d: ParticlesData, box: Box, g_rho, groups=None, want_h=True):
     """Adjoint of compute_density on the context's lists: (dL/dx, dL/dy, dL/dz[, dL/dh]) for
@@ -491,8 +551,9 @@ class HydroStep:
         ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
         sort_by_key(ctx, self.keys, self.order)
         mark(3)
-        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u"):
-            gather(ctx, self.order, inp[f], getattr(d, f))
+        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION:
+            if f in inp:
+                gather(ctx, self.order, inp[f], getattr(d, f))
         mark(4)
         leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(5)
@@ -518,6 +579,19 @@ class HydroStep:
         self.events = ev
         return d
 
+    def integrate(self, ts: TimestepState, remove_unconverged=False):
+        """HydroProp::integrate (main/src/propagator/std_hydro.hpp:170-186): time step, positions
+        and energy, smoothing lengths, on the SFC-ordered fields of the last compute_forces
+        (which must have received x_m1, y_m1, z_m1 and du_m1).  Raises, as the reference does,
+        when a particle is left with at most itself as neighbor, unless remove_unconverged."""
+        ctx, d = self.ctx, self.d
+        compute_timestep(ctx, d, ts)
+        compute_positions(ctx, d, self.box, ts, self.groups)
+        removed = update_smoothing_length(ctx, d, self.keys, self.groups)
+        if removed and not remove_unconverged:
+            raise RuntimeError("Neighbor search did not converge")
+        return ts
+
     def momentum_energy_vjp(self, g_ax=None, g_ay=None, g_az=None, g_du=None, recompute_lists=True):
         """dL/d(x, y, z), in SFC order, of the last compute_forces' momentum-energy outputs for the
         given output gradients (rho, p, c and the IAD tensors held fixed).  recompute_lists re-runs
@@ -534,3 +608,47 @@ class HydroStep:
             return {}
         self.events[-1].synchronize()
         return {s: self.events[k].elapsed_time(self.events[k + 1]) for k, s in enumerate(STAGES)}
+
+
+class Simulation:
+    """Repeated single-rank time steps of the std-hydro propagator: each ``step()`` re-sorts
+    the particles (keys, sort, permutation of every conserved field, as Domain::sync does on one
+    rank, main/src/propagator/std_hydro.hpp:103-116 with ConservedFields :72), evaluates the
+    forces and integrates.  The state stays in HBM in SFC order between steps; two buffer sets
+    alternate so that the permutation never gathers in place."""
+
+    CONSERVED = ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION
+
+    def __init__(self, ctx: Context, state: dict, box: Box, precision=FP64, ts: TimestepState | None = None,
+                 **step_kw):
+        n = state["x"].numel()
+        self.step_obj = HydroStep(ctx, n, box, precision, **step_kw)
+        d = self.step_obj.d
+        d.allocate(ctx.device, integration=True)
+        dt = d.dtype
+        self.state = {}
+        for f in self.CONSERVED:
+            want = torch.float64 if f in ("x", "y", "z") else dt
+            src = state.get(f)
+            self.state[f] = (torch.zeros(n, dtype=want, device=ctx.device) if src is None
+                             else src.to(device=ctx.device, dtype=want).clone())
+        self.ts = ts if ts is not None else TimestepState()
+        self.steps = 0
+
+    @property
+    def d(self) -> ParticlesData:
+        return self.step_obj.d
+
+    def step(self):
+        st = self.step_obj
+        st.compute_forces(self.state)
+        st.integrate(self.ts)
+        # the integrated SFC-ordered fields become the state; the old state buffers receive the
+        # next step's permutation
+        d = st.d
+        for f in self.CONSERVED:
+            cur = getattr(d, f)
+            setattr(d, f, self.state[f])
+            self.state[f] = cur
+        self.steps += 1
+        return self.ts

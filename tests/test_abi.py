@@ -74,6 +74,7 @@ def test_context_without_device():
 MOCK = r"""
 #include <cstddef>
 #include <vector>
+#include <array>
 #include "sphx_compat.hpp"
 enum class BT : char { open = 0, periodic = 1 };
 struct MockBox {
@@ -87,6 +88,8 @@ template<class T> struct DevVec { T* p = nullptr; size_t n = 0; T* data() { retu
 struct Data {
     DevVec<double> x, y, z; DevVec<float> h, m, vx, vy, vz, u, rho, p, c, c11, c12, c13, c22, c23, c33, ax, ay, az, du, wh;
     DevVec<unsigned> nc; unsigned ng0 = 100, ngmax = 150; double K = 0.79, gamma = 5.0 / 3.0, minDtCourant = 0; float Kcour = 0.2f;
+    DevVec<float> x_m1, y_m1, z_m1, du_m1; DevVec<uint64_t> keys;
+    double minDt = 1e-12, minDt_m1 = 1e-12, ttot = 0, minDtRho = 1e30, g = 0, etaAcc = 0.2;
 };
 struct Group { unsigned firstBody, lastBody, numGroups; const unsigned* groupStart; const unsigned* groupEnd; };
 void use(Data& d, const Group& g) {
@@ -95,6 +98,9 @@ void use(Data& d, const Group& g) {
     sphx_compat::computeEOS_HydroStd(0, 10, d);
     sphx_compat::computeIAD(g, d, box);
     sphx_compat::computeMomentumEnergySTD(g, d, box);
+    sphx_compat::computeTimestep(0, 10, d);
+    sphx_compat::computePositions(g, d, box, float(d.minDt), std::array<float, 1>{float(d.minDt_m1)});
+    (void)sphx_compat::updateSmoothingLength(g, d);
     sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
     (void)sphx_compat::computeGroupSplits(0u, 0u, nullptr, nullptr, nullptr, (const double*)nullptr, nullptr, 0,
                                           nullptr, box);
@@ -151,3 +157,21 @@ def test_target_groups_struct_points_into_bounds():
     g = TargetGroups(0, 100, 2, b).struct()
     assert (g.firstBody, g.lastBody, g.numGroups) == (0, 100, 2)
     assert g.groupStart == b.data_ptr() and g.groupEnd == b.data_ptr() + 4
+
+
+@pytest.mark.parametrize("struct,cname", [("Particles", "sphx_particles"), ("Timestep", "sphx_timestep"),
+                                          ("Box", "sphx_box"), ("Octree", "sphx_octree"), ("Groups", "sphx_groups")])
+def test_ctypes_struct_layout_matches_header(struct, cname):
+    """Every field of the ctypes mirror sits at the offset the C compiler gives it."""
+    cls = getattr(_lib, struct)
+    fields = [f[0] for f in cls._fields_]
+    body = "".join(f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields)
+    src = f'#include <stddef.h>\n#include <stdio.h>\n#include "sphx.h"\nint main(void){{{body}printf("%zu\\n", sizeof({cname}));return 0;}}\n'
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, "layout.c")
+        exe = os.path.join(tmp, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
+        out = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    want = [getattr(cls, f).offset for f in fields] + [C.sizeof(cls)]
+    assert out == want

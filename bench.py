@@ -11,7 +11,8 @@ periodic lattice (sigma = 0.1 spacing), velocities a Gaussian random field with
 P(k) ~ k^(-11/3) for 1 <= |k| <= 3 at rms Mach 0.3, u = 1, h for ~100 neighbors.
 `value` is the mixed-precision step's particles/s; the FP64 step is timed in the same
 run and reported beside it.  Also reported: the momentum-energy adjoint with respect to
-positions after a step (lists recomputed), the glass relaxation of a 100^3 jittered
+positions after a step (lists recomputed), the integration step (time step, positions and
+energy, smoothing lengths) that completes a time step, the glass relaxation of a 100^3 jittered
 lattice with the density adjoint (200 iterations), the roofline of the dominant kernel
 (live HIP-event timing; HBM traffic from the committed PMC summary in profiles/), and
 the reference CPU path timed on the host cores on the same input (one step).
@@ -91,6 +92,34 @@ def time_me_adjoint(step, reps=3):
         step.momentum_energy_vjp(*g)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def time_integrate(step, reps=5):
+    """HydroProp::integrate after a force evaluation: computeTimestep (acceleration limit
+    included, g = 1), computePositions and updateSmoothingLength on the step's SFC-ordered
+    fields, with zero x_m1 / du_m1 state.  Each call moves the particles, so it runs after
+    every other measurement on this step."""
+    import torch
+
+    from sphexa_amd import hydro
+
+    d = step.d
+    d.allocate(d.x.device, integration=True)
+    ts = hydro.TimestepState(minDt=1e-6, minDt_m1=1e-6, g=1.0)
+    step.integrate(ts)
+    torch.cuda.synchronize()
+    step.ctx.set_timing(True)
+    step.ctx.timing_summary()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step.integrate(ts)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    kernels = step.ctx.timing_summary()
+    step.ctx.set_timing(False)
+    return {"ms": ms, "kernels_ms": {k: round(v[1] / reps, 4) for k, v in kernels.items()}, "minDt": ts.minDt,
+            "note": "computeTimestep (acceleration limit, one host sync) + computePositions + "
+                    "updateSmoothingLength (one host sync)"}
 
 
 def glass_relaxation(ctx, side=100, iterations=200):
@@ -268,6 +297,8 @@ def main():
         if not args.no_adjoint:
             results[prec]["me_adjoint_ms"] = time_me_adjoint(step)
             log(f"precision {prec}: momentum-energy adjoint {results[prec]['me_adjoint_ms']:.2f} ms")
+        results[prec]["integrate"] = time_integrate(step)
+        log(f"precision {prec}: integrate {results[prec]['integrate']['ms']:.3f} ms")
         del step
         torch.cuda.empty_cache()
 
@@ -299,9 +330,12 @@ def main():
         out["fp64"] = {"value": n / (F["ms"] * 1e-3), "ms_per_step": F["ms"],
                        "stages_ms": {k: round(v, 3) for k, v in F["stages"].items()},
                        "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in F["kernels"].items()}}
+        out["fp64"]["integrate_ms"] = F["integrate"]["ms"]
         out["mixed_over_fp64"] = F["ms"] / R["ms"]
         rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
         out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())
+    out["integrate"] = R["integrate"]
+    out["time_step_ms"] = R["ms"] + R["integrate"]["ms"]
     if "me_adjoint_ms" in R:
         out["momentum_energy_adjoint"] = {
             "ms": R["me_adjoint_ms"], "wrt": "x, y, z",

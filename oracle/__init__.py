@@ -56,6 +56,28 @@ def build(quiet=True):
         print(out.stdout)
 
 
+class VeFields(C.Structure):
+    """Input fields of the VE loops (OrcVe in sphx_oracle.cpp): float64 arrays or None."""
+
+    NAMES = ("x", "y", "z", "vx", "vy", "vz", "h", "m", "c", "xm", "kx", "gradh", "prho", "divv", "alpha", "c11",
+             "c12", "c13", "c22", "c23", "c33", "dV11", "dV12", "dV13", "dV22", "dV23", "dV33", "wh", "whd")
+    _fields_ = [(nm, C.c_void_p) for nm in NAMES] + [("K", C.c_double)]
+
+    @classmethod
+    def of(cls, fields: dict, K: float):
+        keep = {}
+        v = cls()
+        for nm in cls.NAMES:
+            a = fields.get(nm)
+            if a is not None:
+                a = np.ascontiguousarray(a, np.float64)
+                keep[nm] = a
+                setattr(v, nm, a.ctypes.data)
+        v.K = K
+        v._keep = keep  # keeps the arrays alive with the structure
+        return v
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
@@ -120,6 +142,15 @@ class Cpu:
         else:
             f("compute_positions", None, [_U64, _U, _U] + [_V] * 16 + [_D, _D, _V])
         f("update_smoothing_length", _I, [_U, _U, _U, _V, _V, _V])
+        pre = [] if kind == "oracle" else [_U64]
+        loop = pre + [_U, _U, _V, _V, _U, _V, _V]
+        f("ve_xmass", None, loop + [_V])
+        f("ve_def_gradh", None, loop + [_V, _V])
+        f("ve_eos", None, pre + [_U, _U, _V, _V, _V, _V, _V, _D, _V, _V, _V, _V])
+        f("ve_iad_divv_curlv", None, loop + [_I] + [_V] * 14)
+        f("ve_av_switches", None, loop + [_D, _D, _D, _D, _V])
+        f("ve_momentum_energy", _D, loop + [_I, _D, _D, _D, C.c_float, _V, _V, _V, _V])
+        f("kernel_derivative_table", None, [_D, _V])
 
     def _f(self, name, res, args):
         fn = getattr(self.lib, f"{self.pre}_{name}")
@@ -333,6 +364,67 @@ class Cpu:
         last = len(h) if last is None else last
         r = self._update_smoothing_length(first, last, ng0, _p(nc), _p(h), _p(k))
         return h, k, bool(r)
+
+
+    # ---------------------------------------------------------------- VE loops (FP64)
+    def kernel_derivative_table(self, sinc_index=6.0):
+        whd = np.zeros(TABLE_SIZE, np.float64)
+        self._kernel_derivative_table(sinc_index, _p(whd))
+        return whd
+
+    def _ve(self, fields, K, nb, nc, ngmax, first, last):
+        n = len(fields["x"])
+        last = n if last is None else last
+        v = VeFields.of(fields, K)
+        nb = np.ascontiguousarray(nb, np.uint32)
+        nc = np.ascontiguousarray(nc, np.uint32)
+        return n, last, v, [*self._n(n), first, last, _p(nb), _p(nc), ngmax, C.byref(v)], (nb, nc)
+
+    def ve_xmass(self, fields, K, nb, nc, ngmax, box, first=0, last=None):
+        n, last, v, a, keep = self._ve(fields, K, nb, nc, ngmax, first, last)
+        xm = np.zeros(n)
+        self._ve_xmass(*a, C.byref(box), _p(xm))
+        return xm
+
+    def ve_def_gradh(self, fields, K, nb, nc, ngmax, box, first=0, last=None):
+        n, last, v, a, keep = self._ve(fields, K, nb, nc, ngmax, first, last)
+        kx, gradh = np.zeros(n), np.zeros(n)
+        self._ve_def_gradh(*a, C.byref(box), _p(kx), _p(gradh))
+        return kx, gradh
+
+    def ve_eos(self, u, m, kx, xm, gradh, gamma, first=0, last=None):
+        n = len(u)
+        last = n if last is None else last
+        out = [np.zeros(n) for _ in range(4)]
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (u, m, kx, xm, gradh)]
+        self._ve_eos(*self._n(n), first, last, *[_p(a) for a in arrs], gamma, *[_p(o) for o in out])
+        return dict(prho=out[0], c=out[1], rho=out[2], p=out[3])
+
+    def ve_iad_divv_curlv(self, fields, K, nb, nc, ngmax, box, first=0, last=None, grad_v=True, given_iad=False):
+        """IAD (c11..c33) then divv, curlv and the symmetrised velocity gradient; given_iad
+        (oracle only) runs the divv part alone on the c11..c33 of `fields`."""
+        n, last, v, a, keep = self._ve(fields, K, nb, nc, ngmax, first, last)
+        names = ("c11", "c12", "c13", "c22", "c23", "c33", "divv", "curlv", "dV11", "dV12", "dV13", "dV22", "dV23",
+                 "dV33")
+        out = {nm: (np.zeros(n) if (grad_v or not nm.startswith("dV")) else None) for nm in names}
+        self._ve_iad_divv_curlv(*a, C.byref(box), int(given_iad), *[_p(out[nm]) for nm in names])
+        return {k: o for k, o in out.items() if o is not None}
+
+    def ve_av_switches(self, fields, K, nb, nc, ngmax, box, dt, alphamin=0.05, alphamax=1.0, decay_constant=0.2,
+                       first=0, last=None):
+        n, last, v, a, keep = self._ve(fields, K, nb, nc, ngmax, first, last)
+        alpha = np.array(fields["alpha"], dtype=np.float64)
+        self._ve_av_switches(*a, C.byref(box), dt, alphamin, alphamax, decay_constant, _p(alpha))
+        return alpha
+
+    def ve_momentum_energy(self, fields, K, nb, nc, ngmax, box, av_clean=False, Atmin=0.1, Atmax=0.2, ramp=None,
+                           Kcour=0.2, first=0, last=None):
+        n, last, v, a, keep = self._ve(fields, K, nb, nc, ngmax, first, last)
+        ramp = 1.0 / (Atmax - Atmin) if ramp is None else ramp
+        out = [np.zeros(n) for _ in range(4)]
+        dt = self._ve_momentum_energy(*a, C.byref(box), int(av_clean), Atmin, Atmax, ramp, Kcour,
+                                      *[_p(o) for o in out])
+        return out[0], out[1], out[2], out[3], dt
 
 
 def available(kind):

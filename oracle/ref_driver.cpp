@@ -33,6 +33,12 @@
 #include "sph/hydro_std/eos.hpp"
 #include "sph/hydro_std/iad.hpp"
 #include "sph/hydro_std/momentum_energy.hpp"
+#include "sph/hydro_ve/av_switches.hpp"
+#include "sph/hydro_ve/eos.hpp"
+#include "sph/hydro_ve/iad_divv_curlv.hpp"
+#include "sph/hydro_ve/momentum_energy.hpp"
+#include "sph/hydro_ve/ve_def_gradh.hpp"
+#include "sph/hydro_ve/xmass.hpp"
 #include "sph/positions.hpp"
 #include "sph/sph_kernel_tables.hpp"
 #include "sph/update_h.hpp"
@@ -59,6 +65,7 @@ struct Field
     T*     data() const { return p; }
     size_t size() const { return n; }
     T&     operator[](size_t i) const { return p[i]; }
+    bool   empty() const { return n == 0; }
     friend void swap(Field& a, Field& b)
     {
         std::swap(a.p, b.p);
@@ -90,6 +97,34 @@ struct IntData
     using RealType = double;
     Field<double> x, y, z, x_m1, y_m1, z_m1, vx, vy, vz, ax, ay, az, h, u, du, du_m1;
     double        minDt{0}, minDt_m1{0};
+};
+
+// the fields of the VE loops, in the order of the oracle's OrcVe
+struct OrcVe
+{
+    const double *x, *y, *z, *vx, *vy, *vz, *h, *m, *c, *xm, *kx, *gradh, *prho, *divv, *alpha;
+    const double *c11, *c12, *c13, *c22, *c23, *c33;
+    const double *dV11, *dV12, *dV13, *dV22, *dV23, *dV33;
+    const double *wh, *whd;
+    double        K;
+};
+
+// the subset of sphexa::ParticlesData that the VE *Impl templates read
+struct VeData
+{
+    using AcceleratorType = cstone::CpuTag;
+    using HydroType       = double;
+    using RealType        = double;
+
+    Field<double> x, y, z, h, m, vx, vy, vz, u, temp, c, xm, kx, gradh, prho, tdpdTrho, rho, p, divv, curlv, alpha;
+    Field<double> c11, c12, c13, c22, c23, c33, dV11, dV12, dV13, dV22, dV23, dV33, ax, ay, az, du;
+    Field<cstone::LocalIndex> neighbors;
+    Field<unsigned> nc;
+    Field<double> wh, whd;
+    double K{0}, sincIndex{6.0}, gamma{5.0 / 3.0}, muiConst{10.0}, minDtCourant{INFINITY}, minDt{0};
+    double Kcour{0.2}, alphamin{0.05}, alphamax{1.0}, decay_constant{0.2};
+    double Atmin{0.1}, Atmax{0.2}, ramp{10.0};
+    unsigned ngmax{150};
 };
 
 template<class T>
@@ -397,6 +432,104 @@ int ref_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, con
                                 uint64_t* keys)
 {
     return sph::updateSmoothingLengthCpu(first, last, ng0, nc, h, keys) ? 1 : 0;
+}
+
+
+// ---- VE loops: the reference's computeXMassImpl, computeVeDefGradhImpl,
+// computeIdealGasEOS_Impl, computeIadDivvCurlvImpl, computeAVswitchesImpl and
+// computeMomentumEnergyImpl<avClean> (sph/include/sph/hydro_ve/*.hpp), T = double
+static VeData veData(uint64_t n, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax, uint32_t first,
+                     uint32_t last, const OrcVe* f)
+{
+    VeData d;
+    auto   c = [n](const double* p) { return p ? F(p, n) : Field<double>{}; };
+    d.x = c(f->x), d.y = c(f->y), d.z = c(f->z), d.vx = c(f->vx), d.vy = c(f->vy), d.vz = c(f->vz);
+    d.h = c(f->h), d.m = c(f->m), d.c = c(f->c), d.xm = c(f->xm), d.kx = c(f->kx), d.gradh = c(f->gradh);
+    d.prho = c(f->prho), d.divv = c(f->divv), d.alpha = c(f->alpha);
+    d.c11 = c(f->c11), d.c12 = c(f->c12), d.c13 = c(f->c13), d.c22 = c(f->c22), d.c23 = c(f->c23), d.c33 = c(f->c33);
+    d.dV11 = c(f->dV11), d.dV12 = c(f->dV12), d.dV13 = c(f->dV13), d.dV22 = c(f->dV22), d.dV23 = c(f->dV23);
+    d.dV33 = c(f->dV33);
+    d.wh = F(f->wh, size_t(sph::lt::kTableSize)), d.whd = F(f->whd, size_t(sph::lt::kTableSize));
+    d.K = f->K, d.ngmax = ngmax;
+    d.neighbors = F(nbrs, size_t(ngmax) * (last - first));
+    d.nc        = F(nc, n);
+    return d;
+}
+
+void ref_ve_xmass(uint64_t n, uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                  unsigned ngmax, const OrcVe* f, const sphx_box* b, double* xm)
+{
+    VeData d = veData(n, nbrs, nc, ngmax, first, last, f);
+    d.xm     = F(xm, n);
+    sph::computeXMassImpl(first, last, d, toBox(b));
+}
+
+void ref_ve_def_gradh(uint64_t n, uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                      unsigned ngmax, const OrcVe* f, const sphx_box* b, double* kx, double* gradh)
+{
+    VeData d = veData(n, nbrs, nc, ngmax, first, last, f);
+    d.kx = F(kx, n), d.gradh = F(gradh, n);
+    sph::computeVeDefGradhImpl(first, last, d, toBox(b));
+}
+
+void ref_ve_eos(uint64_t n, uint32_t first, uint32_t last, const double* u, const double* m, const double* kx,
+                const double* xm, const double* gradh, double gamma, double* prho, double* c, double* rho, double* p)
+{
+    VeData d;
+    d.u = F(u, n), d.m = F(m, n), d.kx = F(kx, n), d.xm = F(xm, n), d.gradh = F(gradh, n);
+    d.prho = F(prho, n), d.c = F(c, n);
+    if (rho) d.rho = F(rho, n);
+    if (p) d.p = F(p, n);
+    d.gamma = gamma;
+    sph::computeIdealGasEOS_Impl(first, last, d);
+}
+
+void ref_ve_iad_divv_curlv(uint64_t n, uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                           unsigned ngmax, const OrcVe* f, const sphx_box* b, int /*givenIad: not supported*/,
+                           double* c11, double* c12, double* c13,
+                           double* c22, double* c23, double* c33, double* divv, double* curlv, double* dV11,
+                           double* dV12, double* dV13, double* dV22, double* dV23, double* dV33)
+{
+    VeData d = veData(n, nbrs, nc, ngmax, first, last, f);
+    d.c11 = F(c11, n), d.c12 = F(c12, n), d.c13 = F(c13, n), d.c22 = F(c22, n), d.c23 = F(c23, n);
+    d.c33 = F(c33, n), d.divv = F(divv, n);
+    d.curlv = curlv ? F(curlv, n) : Field<double>{};
+    if (dV11)
+        d.dV11 = F(dV11, n), d.dV12 = F(dV12, n), d.dV13 = F(dV13, n), d.dV22 = F(dV22, n), d.dV23 = F(dV23, n),
+        d.dV33 = F(dV33, n);
+    else
+        d.dV11 = d.dV12 = d.dV13 = d.dV22 = d.dV23 = d.dV33 = Field<double>{};
+    sph::computeIadDivvCurlvImpl(first, last, d, toBox(b));
+}
+
+void ref_ve_av_switches(uint64_t n, uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                        unsigned ngmax, const OrcVe* f, const sphx_box* b, double dt, double alphamin,
+                        double alphamax, double decay_constant, double* alpha)
+{
+    VeData d = veData(n, nbrs, nc, ngmax, first, last, f);
+    d.alpha = F(alpha, n);
+    d.minDt = dt, d.alphamin = alphamin, d.alphamax = alphamax, d.decay_constant = decay_constant;
+    sph::computeAVswitchesImpl(first, last, d, toBox(b));
+}
+
+double ref_ve_momentum_energy(uint64_t n, uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                              unsigned ngmax, const OrcVe* f, const sphx_box* b, int avClean, double Atmin,
+                              double Atmax, double ramp, float Kcour, double* gpx, double* gpy, double* gpz,
+                              double* du)
+{
+    VeData d = veData(n, nbrs, nc, ngmax, first, last, f);
+    d.ax = F(gpx, n), d.ay = F(gpy, n), d.az = F(gpz, n), d.du = F(du, n);
+    d.Atmin = Atmin, d.Atmax = Atmax, d.ramp = ramp, d.Kcour = Kcour;
+    if (avClean) sph::computeMomentumEnergyImpl<true>(first, last, d, toBox(b));
+    else sph::computeMomentumEnergyImpl<false>(first, last, d, toBox(b));
+    return d.minDtCourant;
+}
+
+void ref_kernel_derivative_table(double sincIndex, double* whd)
+{
+    auto t = sph::tabulateFunction<double, sph::lt::kTableSize>(
+        sph::getSphKernelDerivative(sph::SphKernelType::sinc_n, sincIndex), 0, 2);
+    std::copy(t.begin(), t.end(), whd);
 }
 
 } // extern "C"

@@ -1044,7 +1044,8 @@ double orc_min_h2_a2(uint32_t first, uint32_t last, const double* h, const doubl
     double m = INFINITY;
     for (uint32_t i = first; i < last; ++i)
     {
-        double a2 = ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]; // norm2, util/array.hpp
+        // norm2 = dot(a, a), a right fold a0 a0 + (a1 a1 + a2 a2) (util/array.hpp:236-240,311-320)
+        double a2 = ax[i] * ax[i] + (ay[i] * ay[i] + az[i] * az[i]);
         m         = std::min(m, h[i] * h[i] / a2);
     }
     return m;
@@ -1152,6 +1153,396 @@ int orc_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, con
         h[i] = orc_update_h(ng0, nc[i], h[i]);
     }
     return removed;
+}
+
+
+// --------------------------------------------------------------------------- VE hydro (T = double)
+// The fields the VE loops read; unused pointers may be NULL.  Layout mirrored by the
+// ctypes structure in oracle/__init__.py.
+struct OrcVe
+{
+    const double *x, *y, *z, *vx, *vy, *vz, *h, *m, *c, *xm, *kx, *gradh, *prho, *divv, *alpha;
+    const double *c11, *c12, *c13, *c22, *c23, *c33;
+    const double *dV11, *dV12, *dV13, *dV22, *dV23, *dV33;
+    const double *wh, *whd;
+    double        K;
+};
+
+// xmassJLoop (sph/include/sph/hydro_ve/xmass_kern.hpp:50-79) over a range
+// (computeXMassImpl, hydro_ve/xmass.hpp:39-67); distancePBC = applyPBC with 2 h
+void orc_ve_xmass(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                  const OrcVe* f, const sphx_box* b, double* xm)
+{
+    Box box = makeBox(b);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + size_t(ngmax) * ni;
+        double xi = f->x[i], yi = f->y[i], zi = f->z[i], hi = f->h[i], mi = f->m[i];
+        double hInv = 1.0 / hi, h3Inv = hInv * hInv * hInv;
+        double rho0i = mi;
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double xx = xi - f->x[j], yy = yi - f->y[j], zz = zi - f->z[j];
+            applyPBC(box, 2.0 * hi, xx, yy, zz);
+            double dist = std::sqrt(xx * xx + yy * yy + zz * zz);
+            rho0i += lookup(f->wh, dist * hInv) * f->m[j];
+        }
+        xm[i] = mi / (rho0i * f->K * h3Inv); // veDefinition (xmass_kern.hpp:45-48)
+    }
+}
+
+// veDefGradhJLoop (hydro_ve/ve_def_gradh_kern.hpp:43-89) over a range
+// (computeVeDefGradhImpl, hydro_ve/ve_def_gradh.hpp:40-79)
+void orc_ve_def_gradh(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                      const OrcVe* f, const sphx_box* b, double* kx, double* gradh)
+{
+    Box box = makeBox(b);
+    const double K = f->K;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + size_t(ngmax) * ni;
+        double xi = f->x[i], yi = f->y[i], zi = f->z[i], hi = f->h[i], mi = f->m[i], xmassi = f->xm[i];
+        double hInv = 1.0 / hi, h3Inv = hInv * hInv * hInv;
+        double kxi = xmassi, whomegai = -3.0 * xmassi, wrho0i = -3.0 * mi;
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double xx = xi - f->x[j], yy = yi - f->y[j], zz = zi - f->z[j];
+            applyPBC(box, 2.0 * hi, xx, yy, zz);
+            double dist  = std::sqrt(xx * xx + yy * yy + zz * zz);
+            double vloc  = dist * hInv;
+            double w     = lookup(f->wh, vloc);
+            double dw    = lookup(f->whd, vloc);
+            double dterh = -(3.0 * w + vloc * dw);
+            double xmj   = f->xm[j];
+            kxi += w * xmj;
+            whomegai += dterh * xmj;
+            wrho0i += dterh * f->m[j];
+        }
+        kxi *= K * h3Inv;
+        whomegai *= K * h3Inv * hInv;
+        wrho0i *= K * h3Inv * hInv;
+        whomegai      = whomegai * mi / xmassi + (kxi - K * xmassi * h3Inv) * wrho0i;
+        double rhoi   = kxi * mi / xmassi;
+        double dhdrho = -hi / (rhoi * 3.0);
+        kx[i]         = kxi;
+        gradh[i]      = 1.0 - dhdrho * whomegai;
+    }
+}
+
+// computeIdealGasEOS_Impl, u branch (hydro_ve/eos.hpp:52-91) with idealGasEOS_u
+// (sph/eos.hpp:34-44); rho and p may be NULL
+void orc_ve_eos(uint32_t first, uint32_t last, const double* u, const double* m, const double* kx,
+                const double* xm, const double* gradh, double gamma, double* prho, double* c, double* rho, double* p)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = first; i < int64_t(last); ++i)
+    {
+        double r   = kx[i] * m[i] / xm[i];
+        double tmp = u[i] * (gamma - 1.0);
+        double pi  = r * tmp;
+        prho[i]    = pi / (kx[i] * m[i] * m[i] * gradh[i]);
+        c[i]       = std::sqrt(gamma * tmp);
+        if (rho) rho[i] = r;
+        if (p) p[i] = pi;
+    }
+}
+
+// computeIadDivvCurlvImpl (hydro_ve/iad_divv_curlv.hpp:40-91): IADJLoop
+// (hydro_ve/iad_kern.hpp:43-110, the std loop with volume xm/kx) then divV_curlVJLoop
+// (hydro_ve/divv_curlv_kern.hpp:43-122) with the fresh c_i; curlv and the dV may be NULL.
+// givenIad != 0 skips the IAD part and reads c11..c33 from f (divV_curlVJLoop on its own,
+// the form of the reference's known-answer test); the outputs c11..c33 are then untouched
+void orc_ve_iad_divv_curlv(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc,
+                           unsigned ngmax, const OrcVe* f, const sphx_box* b, int givenIad, double* c11, double* c12,
+                           double* c13, double* c22, double* c23, double* c33, double* divv, double* curlv,
+                           double* dV11, double* dV12, double* dV13, double* dV22, double* dV23, double* dV33)
+{
+    if (givenIad)
+    {
+        c11 = const_cast<double*>(f->c11), c12 = const_cast<double*>(f->c12), c13 = const_cast<double*>(f->c13);
+        c22 = const_cast<double*>(f->c22), c23 = const_cast<double*>(f->c23), c33 = const_cast<double*>(f->c33);
+    }
+    else
+        orc_iad(first, last, nbrs, nc, ngmax, f->x, f->y, f->z, f->h, f->xm, f->kx, f->wh, f->K, b, c11, c12, c13,
+                c22, c23, c33);
+    Box box = makeBox(b);
+    const double K = f->K;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + size_t(ngmax) * ni;
+        double xi = f->x[i], yi = f->y[i], zi = f->z[i], vxi = f->vx[i], vyi = f->vy[i], vzi = f->vz[i];
+        double hi = f->h[i], kxi = f->kx[i];
+        double hiInv = 1.0 / hi, hiInv3 = hiInv * hiInv * hiInv;
+        double dVx[3] = {0, 0, 0}, dVy[3] = {0, 0, 0}, dVz[3] = {0, 0, 0};
+        double a11 = c11[i], a12 = c12[i], a13 = c13[i], a22 = c22[i], a23 = c23[i], a33 = c33[i];
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double rx = xi - f->x[j], ry = yi - f->y[j], rz = zi - f->z[j];
+            applyPBC(box, 2.0 * hi, rx, ry, rz);
+            double r2   = rx * rx + ry * ry + rz * rz;
+            double dist = std::sqrt(r2);
+            double vxji = f->vx[j] - vxi, vyji = f->vy[j] - vyi, vzji = f->vz[j] - vzi;
+            double Wi   = lookup(f->wh, dist * hiInv);
+            double tA[3];
+            tA[0] = -(a11 * rx + a12 * ry + a13 * rz) * Wi;
+            tA[1] = -(a12 * rx + a22 * ry + a23 * rz) * Wi;
+            tA[2] = -(a13 * rx + a23 * ry + a33 * rz) * Wi;
+            double xmj = f->xm[j];
+            double sx = vxji * xmj, sy = vyji * xmj, sz = vzji * xmj;
+            for (int k = 0; k < 3; ++k)
+            {
+                dVx[k] += sx * tA[k];
+                dVy[k] += sy * tA[k];
+                dVz[k] += sz * tA[k];
+            }
+        }
+        double nk = K * hiInv3 / kxi;
+        divv[i]   = nk * (dVx[0] + dVy[1] + dVz[2]);
+        if (curlv)
+        {
+            double cv0 = dVz[1] - dVy[2], cv1 = dVx[2] - dVz[0], cv2 = dVy[0] - dVx[1];
+            curlv[i]   = nk * std::sqrt(cv0 * cv0 + (cv1 * cv1 + cv2 * cv2)); // norm2: right fold
+        }
+        if (dV11)
+        {
+            dV11[i] = nk * dVx[0];
+            dV12[i] = nk * (dVx[1] + dVy[0]);
+            dV13[i] = nk * (dVx[2] + dVz[0]);
+            dV22[i] = nk * dVy[1];
+            dV23[i] = nk * (dVy[2] + dVz[1]);
+            dV33[i] = nk * dVz[2];
+        }
+    }
+}
+
+// AVswitchesJLoop (hydro_ve/av_switches_kern.hpp:43-136) over a range
+// (computeAVswitchesImpl, hydro_ve/av_switches.hpp:40-78); alpha updated in place
+void orc_ve_av_switches(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                        const OrcVe* f, const sphx_box* b, double dt, double alphamin, double alphamax,
+                        double decay_constant, double* alpha)
+{
+    Box box = makeBox(b);
+    const double K = f->K;
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + size_t(ngmax) * ni;
+        double xi = f->x[i], yi = f->y[i], zi = f->z[i], vxi = f->vx[i], vyi = f->vy[i], vzi = f->vz[i];
+        double hi = f->h[i], ci = f->c[i];
+        double a11 = f->c11[i], a12 = f->c12[i], a13 = f->c13[i], a22 = f->c22[i], a23 = f->c23[i], a33 = f->c33[i];
+        double vsig   = 1.e-40 * ci;
+        double hiInv  = 1.0 / hi, hiInv3 = hiInv * hiInv * hiInv;
+        double divv_i = f->divv[i];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double rx = xi - f->x[j], ry = yi - f->y[j], rz = zi - f->z[j];
+            applyPBC(box, 2.0 * hi, rx, ry, rz);
+            double r2   = rx * rx + ry * ry + rz * rz;
+            double dist = std::sqrt(r2);
+            double vxij = vxi - f->vx[j], vyij = vyi - f->vy[j], vzij = vzi - f->vz[j];
+            double rv   = rx * vxij + ry * vyij + rz * vzij;
+            double vs   = 0.0;
+            if (rv < 0.0) { vs = ci + f->c[j] - 3.0 * rv / dist; }
+            vsig = (vsig < vs) ? vs : vsig; // stl::max
+            double Wi     = K * hiInv3 * lookup(f->wh, dist * hiInv);
+            double tA1    = -(a11 * rx + a12 * ry + a13 * rz) * Wi;
+            double tA2    = -(a12 * rx + a22 * ry + a23 * rz) * Wi;
+            double tA3    = -(a13 * rx + a23 * ry + a33 * rz) * Wi;
+            double volj   = f->xm[j] / f->kx[j];
+            double factor = volj * (divv_i - f->divv[j]);
+            gx += factor * tA1;
+            gy += factor * tA2;
+            gz += factor * tA3;
+        }
+        double graddivv = std::sqrt(gx * gx + gy * gy + gz * gz);
+        double alphaloc = 0.0;
+        if (divv_i < 0.0)
+        {
+            double a_const = hi * hi * graddivv;
+            alphaloc       = alphamax * a_const / (a_const + hi * std::abs(divv_i) + 0.05 * ci);
+        }
+        double alpha_i = alpha[i];
+        if (alphaloc >= alpha_i) { alpha_i = alphaloc; }
+        else
+        {
+            double decay    = hi / (decay_constant * vsig);
+            double alphadot = 0.0;
+            if (alphaloc >= alphamin) { alphadot = (alphaloc - alpha_i) / decay; }
+            else { alphadot = (alphamin - alpha_i) / decay; }
+            alpha_i += alphadot * dt;
+        }
+        alpha[i] = alpha_i;
+    }
+}
+
+// avRvCorrection (hydro_ve/momentum_energy_kern.hpp:43-63) with symv (sph/kernels.hpp)
+static double avRvCorrection(const double R[3], double eta_ab, double eta_crit, const double gi[6], const double gj[6])
+{
+    auto dotSymv = [&](const double g[6]) {
+        double s0 = g[0] * R[0] + g[1] * R[1] + g[2] * R[2];
+        double s1 = g[3] * R[1] + g[4] * R[2];
+        double s2 = g[5] * R[2];
+        return R[0] * s0 + (R[1] * s1 + R[2] * s2); // dot: right fold (util/array.hpp:236-240)
+    };
+    double dmy1 = dotSymv(gi), dmy2 = dotSymv(gj), dmy3 = 1.0;
+    if (eta_ab < eta_crit)
+    {
+        double etaDiff = 5.0 * (eta_ab - eta_crit);
+        dmy3           = std::exp(-etaDiff * etaDiff);
+    }
+    double A_ab   = (dmy2 != 0.0) ? dmy1 / dmy2 : 0.0;
+    double A_abp1 = 1.0 + A_ab;
+    double mn     = 4.0 * A_ab / (A_abp1 * A_abp1);
+    mn            = (mn < 1.0) ? mn : 1.0; // stl::min(1, .)
+    double mx     = (0.0 < mn) ? mn : 0.0; // stl::max(0, .)
+    double phi_ab = 0.5 * dmy3 * mx;
+    return -phi_ab * (dmy1 + dmy2);
+}
+
+// momentumAndEnergyJLoop (hydro_ve/momentum_energy_kern.hpp:65-221, tdpdTrho absent) over a
+// range (computeMomentumEnergyImpl, hydro_ve/momentum_energy.hpp:40-100); returns minDtCourant
+double orc_ve_momentum_energy(uint32_t first, uint32_t last, const uint32_t* nbrs, const unsigned* nc, unsigned ngmax,
+                              const OrcVe* f, const sphx_box* b, int avClean, double Atmin, double Atmax,
+                              double ramp, float Kcour, double* gpx, double* gpy, double* gpz, double* du)
+{
+    Box box = makeBox(b);
+    const double K = f->K;
+    double minDt = INFINITY;
+#pragma omp parallel for schedule(static) reduction(min : minDt)
+    for (int64_t ii = int64_t(first); ii < int64_t(last); ++ii)
+    {
+        size_t i = size_t(ii), ni = i - first;
+        unsigned cnt = std::min(nc[i] - 1, ngmax);
+        const uint32_t* nb = nbrs + size_t(ngmax) * ni;
+        double xi = f->x[i], yi = f->y[i], zi = f->z[i], vxi = f->vx[i], vyi = f->vy[i], vzi = f->vz[i];
+        double hi = f->h[i], mi = f->m[i], ci = f->c[i], kxi = f->kx[i], alpha_i = f->alpha[i];
+        double xmassi = f->xm[i];
+        double rhoi   = kxi * mi / xmassi;
+        double prhoi  = f->prho[i];
+        double hiInv  = 1.0 / hi, hiInv3 = hiInv * hiInv * hiInv;
+        double maxv = 0.0, mx = 0.0, my = 0.0, mz = 0.0, energy = 0.0, avisc_e = 0.0;
+        double a11 = f->c11[i], a12 = f->c12[i], a13 = f->c13[i], a22 = f->c22[i], a23 = f->c23[i], a33 = f->c33[i];
+        double gi[6] = {0, 0, 0, 0, 0, 0};
+        if (avClean) gi[0] = f->dV11[i], gi[1] = f->dV12[i], gi[2] = f->dV13[i], gi[3] = f->dV22[i],
+                     gi[4] = f->dV23[i], gi[5] = f->dV33[i];
+        double eta_crit = std::cbrt(32.0 * M_PI / 3.0 / double(cnt + 1));
+        for (unsigned pj = 0; pj < cnt; ++pj)
+        {
+            uint32_t j = nb[pj];
+            double rx = xi - f->x[j], ry = yi - f->y[j], rz = zi - f->z[j];
+            double vxj = f->vx[j], vyj = f->vy[j], vzj = f->vz[j];
+            applyPBC(box, 2.0 * hi, rx, ry, rz);
+            double r2   = rx * rx + ry * ry + rz * rz;
+            double dist = std::sqrt(r2);
+            double vxij = vxi - vxj, vyij = vyi - vyj, vzij = vzi - vzj;
+            double hj = f->h[j], hjInv = 1.0 / hj;
+            double v1 = dist * hiInv, v2 = dist * hjInv;
+            double hjInv3 = hjInv * hjInv * hjInv;
+            double Wi = hiInv3 * lookup(f->wh, v1);
+            double Wj = hjInv3 * lookup(f->wh, v2);
+            double t1i = -(a11 * rx + a12 * ry + a13 * rz) * Wi;
+            double t2i = -(a12 * rx + a22 * ry + a23 * rz) * Wi;
+            double t3i = -(a13 * rx + a23 * ry + a33 * rz) * Wi;
+            double b11 = f->c11[j], b12 = f->c12[j], b13 = f->c13[j], b22 = f->c22[j], b23 = f->c23[j], b33 = f->c33[j];
+            double t1j = -(b11 * rx + b12 * ry + b13 * rz) * Wj;
+            double t2j = -(b12 * rx + b22 * ry + b23 * rz) * Wj;
+            double t3j = -(b13 * rx + b23 * ry + b33 * rz) * Wj;
+            double mj = f->m[j], cj = f->c[j], kxj = f->kx[j], xmassj = f->xm[j];
+            double rhoj = kxj * mj / xmassj;
+            double rv   = rx * vxij + ry * vyij + rz * vzij;
+            if (avClean)
+            {
+                double R[3]  = {rx, ry, rz};
+                double gj[6] = {f->dV11[j], f->dV12[j], f->dV13[j], f->dV22[j], f->dV23[j], f->dV33[j]};
+                rv += avRvCorrection(R, (v2 < v1) ? v2 : v1, eta_crit, gi, gj);
+            }
+            double wij = rv / dist;
+            // artificial_viscosity (sph/kernels.hpp:60-76), beta = 2
+            double visc = 0.0;
+            if (wij < 0.0)
+            {
+                double vij_signal = (alpha_i + f->alpha[j]) * 0.25 * (ci + cj) - 2.0 * wij;
+                visc              = -vij_signal * wij;
+            }
+            double vijsignal = 0.5 * (ci + cj) - 2.0 * wij;
+            maxv             = (vijsignal > maxv) ? vijsignal : maxv;
+            double a_mom, b_mom;
+            double Atwood = (std::abs(rhoi - rhoj)) / (rhoi + rhoj);
+            if (Atwood < Atmin)
+            {
+                a_mom = xmassi * xmassi;
+                b_mom = xmassj * xmassj;
+            }
+            else if (Atwood > Atmax)
+            {
+                a_mom = xmassi * xmassj;
+                b_mom = a_mom;
+            }
+            else
+            {
+                double sigma_ij = ramp * (Atwood - Atmin);
+                a_mom           = std::pow(xmassi, 2.0 - sigma_ij) * std::pow(xmassj, sigma_ij);
+                b_mom           = std::pow(xmassj, 2.0 - sigma_ij) * std::pow(xmassi, sigma_ij);
+            }
+            double a_visc = mj / rhoi * visc, b_visc = mj / rhoj * visc;
+            double avx = 0.5 * (a_visc * t1i + b_visc * t1j);
+            double avy = 0.5 * (a_visc * t2i + b_visc * t2j);
+            double avz = 0.5 * (a_visc * t3i + b_visc * t3j);
+            avisc_e += avx * vxij + avy * vyij + avz * vzij;
+            energy += mj * a_mom * (vxij * t1i + vyij * t2i + vzij * t3i);
+            double mom_i = mj * prhoi * a_mom;
+            double mom_j = mj * f->prho[j] * b_mom;
+            mx += mom_i * t1i + mom_j * t1j + avx;
+            my += mom_i * t2i + mom_j * t2j + avy;
+            mz += mom_i * t3i + mom_j * t3j + avz;
+        }
+        avisc_e = (0.0 < avisc_e) ? avisc_e : 0.0;
+        du[i]   = K * (prhoi * energy + 0.5 * avisc_e);
+        gpx[i]  = -K * mx;
+        gpy[i]  = -K * my;
+        gpz[i]  = -K * mz;
+        // tsKCourant (sph/kernels.hpp:10-17)
+        double v  = maxv > 0.0 ? maxv : ci;
+        double dt = hi > 0.0 ? double(Kcour * hi / v) : INFINITY;
+        minDt     = std::min(minDt, dt);
+    }
+    return minDt;
+}
+
+// tabulateFunction of getSphKernelDerivative(sinc_n) (sph_kernel_tables.hpp:88-158,
+// kernels.hpp:45-58): whd[k] = n sinc(pi v / 2)^(n-1) d/dv sinc(pi v / 2)
+void orc_kernel_derivative_table(double sincIndex, double* whd)
+{
+    const int    numIntervals = kTableSize - 1;
+    const double dx           = 2.0 / numIntervals;
+    for (int k = 0; k < kTableSize; ++k)
+    {
+        double v = 0.0 + double(size_t(k)) * dx;
+        double d = 0.0;
+        if (v != 0.0)
+        {
+            const double Pv    = M_PI_2 * v;
+            const double sincv = std::sin(Pv) / Pv;
+            d                  = sincv * M_PI_2 * ((std::cos(Pv) / std::sin(Pv)) - 1.0 / Pv);
+        }
+        whd[k] = sincIndex * std::pow(wharmonic(v), sincIndex - 1) * d;
+    }
 }
 
 } // extern "C"

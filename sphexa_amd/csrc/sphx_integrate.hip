@@ -129,7 +129,8 @@ __global__ __launch_bounds__(256) void minH2A2Kernel(uint32_t first, uint32_t la
          i += uint64_t(gridDim.x) * blockDim.x)
     {
         double a0 = ax[i], a1 = ay[i], a2 = az[i];
-        double v  = double(T(h[i] * h[i])) / (a0 * a0 + a1 * a1 + a2 * a2);
+        // norm2 = dot(a, a), a right fold: a0 a0 + (a1 a1 + a2 a2) (cstone/util/array.hpp:236-240)
+        double v  = double(T(h[i] * h[i])) / (a0 * a0 + (a1 * a1 + a2 * a2));
         if (v < m) m = v;
     }
     for (int o = 32; o > 0; o >>= 1)

@@ -312,6 +312,61 @@ int sphx_compute_positions(sphx_ctx* ctx, const sphx_groups* groups, sphx_partic
 int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, uint64_t* keys,
                                  int* keysRemoved);
 
+/* ---- VE hydro (the reference's default propagator) ------------------------- */
+/* The fields of the volume-element (VE) loops that sphx_particles does not hold
+ * (sph/include/sph/particles_data.hpp:218-246); every void* is double* for SPHX_FP64
+ * and float* for SPHX_MIXED, as in sphx_particles.  Outputs of one loop are inputs of
+ * the next, in the order of HydroVeProp::computeForces (main/src/propagator/
+ * ve_hydro.hpp:126-205): xmass -> veDefGradh -> EOS -> iadDivvCurlv -> AVswitches ->
+ * momentum-energy.  dV11..dV33 (the symmetrised velocity gradient) and curlv are
+ * optional (NULL: not computed). */
+typedef struct sphx_ve_fields
+{
+    void *      xm, *kx, *gradh, *prho, *divv, *curlv, *alpha;
+    void *      dV11, *dV12, *dV13, *dV22, *dV23, *dV33;
+    const void* whd;                          /* kernel-derivative table, SPHX_TABLE_SIZE entries */
+    double      alphamin, alphamax, decay;    /* AV switches (particles_data.hpp:127-129)         */
+    double      Atmin, Atmax, ramp;           /* Atwood ramp (particles_data.hpp:134-136)         */
+} sphx_ve_fields;
+
+/* Host-side derivative table: tabulateFunction of getSphKernelDerivative(sinc_n)
+ * (sph/include/sph/sph_kernel_tables.hpp:88-102,161-172), the whd of createTables
+ * (particles_data.hpp:377-387), in double and float; either pointer may be NULL. */
+int sphx_kernel_derivative_tables(double sincIndex, double* whd64, float* whd32);
+
+/* sph::computeXMass (sph/include/sph/hydro_ve/xmass.hpp:39-75, xmassJLoop
+ * xmass_kern.hpp:50-79): xm_i = m_i / (K/h_i^3 (m_i + sum_j m_j W(r_ij/h_i))). */
+int sphx_compute_xmass(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                       const sphx_box* box);
+
+/* sph::computeVeDefGradh (hydro_ve/ve_def_gradh.hpp:40-87, veDefGradhJLoop
+ * ve_def_gradh_kern.hpp:43-89): the VE normalisation kx and the grad-h term gradh. */
+int sphx_compute_ve_def_gradh(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                              const sphx_box* box);
+
+/* sph::computeEOS, ideal gas with u (hydro_ve/eos.hpp:52-91,192-197): rho = kx m / xm,
+ * p = rho (gamma - 1) u, c = sqrt(gamma (gamma - 1) u), prho = p / (kx m^2 gradh);
+ * writes ve->prho and d->c, and d->rho, d->p when they are not NULL. */
+int sphx_compute_eos_ve(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d, sphx_ve_fields* ve);
+
+/* sph::computeIadDivvCurlv (hydro_ve/iad_divv_curlv.hpp:40-98): the IAD tensor with the
+ * VE volume xm/kx (IADJLoop, hydro_ve/iad_kern.hpp:43-110) into d->c11..c33, then
+ * divv, curlv and the symmetrised velocity gradient (divV_curlVJLoop,
+ * divv_curlv_kern.hpp:43-122) with those tensors. */
+int sphx_compute_iad_divv_curlv(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                                const sphx_box* box);
+
+/* sph::computeAVswitches (hydro_ve/av_switches.hpp:40-88, AVswitchesJLoop
+ * av_switches_kern.hpp:43-136): updates ve->alpha in place over the time step dt. */
+int sphx_compute_av_switches(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                             const sphx_box* box, double dt);
+
+/* sph::computeMomentumEnergy<avClean = false> (hydro_ve/momentum_energy.hpp:40-117,
+ * momentumAndEnergyJLoop momentum_energy_kern.hpp:65-221, tdpdTrho absent): ax, ay, az,
+ * du and d->minDtCourant.  Synchronises the stream once to return minDtCourant. */
+int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
+                                    sphx_ve_fields* ve, const sphx_box* box);
+
 /* ---- adjoints -------------------------------------------------------------- */
 /* No counterpart in the reference (its loops are forward-only).  Vector-Jacobian
  * product of sphx_compute_density with respect to the positions and h, on the lists

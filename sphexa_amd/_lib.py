@@ -58,6 +58,12 @@ class Particles(C.Structure):
                 ("wh", _V), ("minDtCourant", _D), ("x_m1", _V), ("y_m1", _V), ("z_m1", _V), ("du_m1", _V)]
 
 
+class VeFields(C.Structure):
+    _fields_ = [(nm, _V) for nm in ("xm", "kx", "gradh", "prho", "divv", "curlv", "alpha", "dV11", "dV12", "dV13",
+                                    "dV22", "dV23", "dV33", "whd")] + \
+               [(nm, _D) for nm in ("alphamin", "alphamax", "decay", "Atmin", "Atmax", "ramp")]
+
+
 class Timestep(C.Structure):
     _fields_ = [("minDt", _D), ("minDt_m1", _D), ("ttot", _D), ("minDtCourant", _D), ("minDtRho", _D), ("g", _D),
                 ("etaAcc", _D), ("minDtAcc", _D)]
@@ -103,6 +109,18 @@ SIGNATURES = {
                                          C.POINTER(Timestep)]),
     "sphx_update_smoothing_length": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), _V,
                                                C.POINTER(C.c_int)]),
+    "sphx_kernel_derivative_tables": (C.c_int, [_D, _V, _V]),
+    "sphx_compute_xmass": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),
+                                     C.POINTER(Box)]),
+    "sphx_compute_ve_def_gradh": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),
+                                            C.POINTER(Box)]),
+    "sphx_compute_eos_ve": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles), C.POINTER(VeFields)]),
+    "sphx_compute_iad_divv_curlv": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),
+                                              C.POINTER(Box)]),
+    "sphx_compute_av_switches": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),
+                                           C.POINTER(Box), _D]),
+    "sphx_compute_momentum_energy_ve": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles),
+                                                  C.POINTER(VeFields), C.POINTER(Box)]),
     "sphx_make_global_box": (C.c_int, [_V, _V, _V, _V, _U64, C.POINTER(Box), C.POINTER(Box)]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),

@@ -10,6 +10,7 @@
 #define SPHX_HYDRO_NS fast
 #define SPHX_HYDRO_EXACT 0
 #include "sphx_hydro_kernels.inc"
+#include "sphx_ve_kernels.inc"
 #include "sphx_adjoint.inc"
 
 namespace sphx
@@ -20,6 +21,14 @@ int launchDensity(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*,
 int launchIAD(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&);
 int launchMomentumEnergy(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&,
                          unsigned long long*);
+int launchXmass(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*, const DBox&);
+int launchVeDefGradh(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*, const DBox&);
+int launchIadDivvCurlv(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*,
+                       const DBox&);
+int launchAvSwitches(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*, const DBox&,
+                     double);
+int launchMomentumEnergyVe(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*,
+                           const DBox&, unsigned long long*);
 } // namespace exact
 
 int groupsFromArgs(sphx_ctx* ctx, const sphx_groups* groups, uint32_t first, uint32_t last, GroupsDev& G);
@@ -233,6 +242,158 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
     SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt)
                               : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt));
+    unsigned long long bits = 0;
+    SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    d->minDtCourant = v;
+    return SPHX_OK;
+}
+
+int sphx_kernel_derivative_tables(double sincIndex, double* whd64, float* whd32)
+{
+#pragma clang fp contract(off)
+    // powSincDerivative (sph_kernel_tables.hpp:104-108) with wharmonic_derivative_std (kernels.hpp:45-58)
+    auto dW = [sincIndex](double v) {
+        double s = 1.0, d = 0.0;
+        if (v != 0.0)
+        {
+            const double Pv = M_PI_2 * v;
+            s               = std::sin(Pv) / Pv;
+            d               = s * M_PI_2 * ((std::cos(Pv) / std::sin(Pv)) - 1.0 / Pv);
+        }
+        return sincIndex * std::pow(s, sincIndex - 1) * d;
+    };
+    const int numIntervals = SPHX_TABLE_SIZE - 1;
+    if (whd64)
+    {
+        const double dx = 2.0 / numIntervals;
+        for (int i = 0; i < SPHX_TABLE_SIZE; ++i)
+            whd64[i] = dW(0.0 + double(size_t(i)) * dx);
+    }
+    if (whd32)
+    {
+        const float dx = float(2.0 / numIntervals);
+        for (int i = 0; i < SPHX_TABLE_SIZE; ++i)
+            whd32[i] = float(dW(double(float(0.0 + double(float(size_t(i)) * dx)))));
+    }
+    return SPHX_OK;
+}
+
+static int checkVe(sphx_ctx* ctx, const sphx_ve_fields* ve, const char* what)
+{
+    if (!ve) return argError(ctx, what);
+    return SPHX_OK;
+}
+
+int sphx_compute_xmass(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                       const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_xmass: null field"));
+    SPHX_TRY(checkVe(ctx, ve, "compute_xmass: null ve fields"));
+    if (!ve->xm) return argError(ctx, "compute_xmass: null xm");
+    if (!validBox(box)) return argError(ctx, "compute_xmass: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    DBox db = makeDBox(box);
+    return useExact(ctx, d) ? exact::launchXmass(ctx, G, lists, d, ve, db) : fast::launchXmass(ctx, G, lists, d, ve, db);
+}
+
+int sphx_compute_ve_def_gradh(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                              const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_ve_def_gradh: null field"));
+    SPHX_TRY(checkVe(ctx, ve, "compute_ve_def_gradh: null ve fields"));
+    if (!ve->xm || !ve->kx || !ve->gradh || (d->precision == SPHX_FP64 && !ve->whd))
+        return argError(ctx, "compute_ve_def_gradh: null xm, kx, gradh or whd");
+    if (!validBox(box)) return argError(ctx, "compute_ve_def_gradh: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    DBox db = makeDBox(box);
+    return useExact(ctx, d) ? exact::launchVeDefGradh(ctx, G, lists, d, ve, db)
+                            : fast::launchVeDefGradh(ctx, G, lists, d, ve, db);
+}
+
+int sphx_compute_eos_ve(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d, sphx_ve_fields* ve)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!d || !d->u || !d->m || !d->c) return argError(ctx, "compute_eos_ve: null field");
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED)
+        return argError(ctx, "compute_eos_ve: unknown precision");
+    SPHX_TRY(checkVe(ctx, ve, "compute_eos_ve: null ve fields"));
+    if (!ve->xm || !ve->kx || !ve->gradh || !ve->prho) return argError(ctx, "compute_eos_ve: null ve field");
+    if (last < first || last > d->n) return argError(ctx, "compute_eos_ve: bad range");
+    if (last == first) return SPHX_OK;
+    return fast::launchEosVe(ctx, first, last, d, ve);
+}
+
+int sphx_compute_iad_divv_curlv(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                                const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_iad_divv_curlv: null field"));
+    SPHX_TRY(checkVe(ctx, ve, "compute_iad_divv_curlv: null ve fields"));
+    if (!d->vx || !d->vy || !d->vz || !d->c11 || !d->c12 || !d->c13 || !d->c22 || !d->c23 || !d->c33 || !ve->xm ||
+        !ve->kx || !ve->divv)
+        return argError(ctx, "compute_iad_divv_curlv: null field");
+    if (!validBox(box)) return argError(ctx, "compute_iad_divv_curlv: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    DBox db = makeDBox(box);
+    return useExact(ctx, d) ? exact::launchIadDivvCurlv(ctx, G, lists, d, ve, db)
+                            : fast::launchIadDivvCurlv(ctx, G, lists, d, ve, db);
+}
+
+int sphx_compute_av_switches(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
+                             const sphx_box* box, double dt)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_av_switches: null field"));
+    SPHX_TRY(checkVe(ctx, ve, "compute_av_switches: null ve fields"));
+    if (!d->vx || !d->vy || !d->vz || !d->c || !d->c11 || !d->c12 || !d->c13 || !d->c22 || !d->c23 || !d->c33 ||
+        !ve->xm || !ve->kx || !ve->divv || !ve->alpha)
+        return argError(ctx, "compute_av_switches: null field");
+    if (!validBox(box)) return argError(ctx, "compute_av_switches: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    if (G.numGroups == 0) return SPHX_OK;
+    DBox db = makeDBox(box);
+    return useExact(ctx, d) ? exact::launchAvSwitches(ctx, G, lists, d, ve, db, dt)
+                            : fast::launchAvSwitches(ctx, G, lists, d, ve, db, dt);
+}
+
+int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
+                                    sphx_ve_fields* ve, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_momentum_energy_ve: null field"));
+    SPHX_TRY(checkVe(ctx, ve, "compute_momentum_energy_ve: null ve fields"));
+    if (!d->vx || !d->vy || !d->vz || !d->c || !d->c11 || !d->c12 || !d->c13 || !d->c22 || !d->c23 || !d->c33 ||
+        !d->ax || !d->ay || !d->az || !d->du || !ve->xm || !ve->kx || !ve->prho || !ve->alpha)
+        return argError(ctx, "compute_momentum_energy_ve: null field");
+    if (!validBox(box)) return argError(ctx, "compute_momentum_energy_ve: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    d->minDtCourant = INFINITY;
+    if (G.numGroups == 0) return SPHX_OK;
+    auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
+    if (!dt) return SPHX_ERR_NOMEM;
+    DBox db = makeDBox(box);
+    initMinDt<<<1, 64, 0, ctx->stream>>>(dt);
+    SPHX_LAUNCH_CHECK(ctx, "initMinDt");
+    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt)
+                              : fast::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt));
     unsigned long long bits = 0;
     SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
     SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));

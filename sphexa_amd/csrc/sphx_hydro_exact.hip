@@ -5,3 +5,4 @@
 #define SPHX_HYDRO_NS exact
 #define SPHX_HYDRO_EXACT 1
 #include "sphx_hydro_kernels.inc"
+#include "sphx_ve_kernels.inc"

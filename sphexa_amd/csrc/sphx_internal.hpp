@@ -44,6 +44,7 @@ enum Slot : int
     S_SMALL,         // small device scalars (flags, counters, reductions)
     S_TABLE64,       // sinc table, FP64
     S_TABLE32,       // sinc table, FP32
+    S_TABLE_D,       // kernel-derivative table (VE gradh loop), FP64
     S_HY_REC,        // packed neighbor-side records of the SPH loops
     S_NUM_SLOTS
 };

@@ -24,6 +24,13 @@ this module                    reference entry point
 ``HydroStep.compute_forces``   HydroProp::computeForces (main/src/propagator/std_hydro.hpp:119)
 ``HydroStep.integrate``        HydroProp::integrate (main/src/propagator/std_hydro.hpp:170)
 ``Simulation.step``            one time step: sync, computeForces, integrate
+``compute_xmass``              sph::computeXMass (sph/include/sph/hydro_ve/xmass.hpp:69)
+``compute_ve_def_gradh``       sph::computeVeDefGradh (hydro_ve/ve_def_gradh.hpp:80)
+``compute_eos_ve``             sph::computeEOS, ideal gas (hydro_ve/eos.hpp:192)
+``compute_iad_divv_curlv``     sph::computeIadDivvCurlv (hydro_ve/iad_divv_curlv.hpp:92)
+``compute_av_switches``        sph::computeAVswitches (hydro_ve/av_switches.hpp:81)
+``compute_momentum_energy_ve`` sph::computeMomentumEnergy<false> (hydro_ve/momentum_energy.hpp:104)
+``VeHydroStep.compute_forces`` HydroVeProp::computeForces (main/src/propagator/ve_hydro.hpp:126)
 =============================  ==============================================================
 
 Errors raise ``SphxError`` carrying the library's status code; there is no CPU
@@ -40,7 +47,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import Box, Groups, Octree, Particles, SphxError, Timestep, FP64, MIXED, BC_OPEN, BC_PERIODIC, BC_FIXED
+from ._lib import Box, Groups, Octree, Particles, SphxError, Timestep, VeFields, FP64, MIXED, BC_OPEN, BC_PERIODIC, \
+    BC_FIXED
 
 _hip = None
 
@@ -479,6 +487,108 @@ def update_smoothing_length(ctx: Context, d: ParticlesData, keys=None, groups=No
     return bool(r.value)
 
 
+# ------------------------------------------------------------------------------------------- VE hydro
+def kernel_derivative_tables(sinc_index: float = 6.0):
+    """(whd64 numpy, whd32 numpy): the kernel-derivative table of createTables."""
+    lib = _lib.load()
+    whd64 = np.zeros(_lib.TABLE_SIZE, np.float64)
+    whd32 = np.zeros(_lib.TABLE_SIZE, np.float32)
+    lib.sphx_kernel_derivative_tables(sinc_index, whd64.ctypes.data_as(C.c_void_p), whd32.ctypes.data_as(C.c_void_p))
+    return whd64, whd32
+
+
+@dataclass
+class VeData:
+    """Device fields of the VE loops that ParticlesData does not hold (particles_data.hpp:218-246)
+    and the VE constants (alphamin, alphamax, decay_constant :127-129; Atmin, Atmax, ramp
+    :134-136).  grad_v allocates the symmetrised velocity gradient dV11..dV33."""
+
+    n: int
+    precision: int = FP64
+    alphamin: float = 0.05
+    alphamax: float = 1.0
+    decay: float = 0.2
+    Atmin: float = 0.1
+    Atmax: float = 0.2
+    ramp: float = 1.0 / (0.2 - 0.1)
+    xm: torch.Tensor | None = None
+    kx: torch.Tensor | None = None
+    gradh: torch.Tensor | None = None
+    prho: torch.Tensor | None = None
+    divv: torch.Tensor | None = None
+    curlv: torch.Tensor | None = None
+    alpha: torch.Tensor | None = None
+    dV11: torch.Tensor | None = None
+    dV12: torch.Tensor | None = None
+    dV13: torch.Tensor | None = None
+    dV22: torch.Tensor | None = None
+    dV23: torch.Tensor | None = None
+    dV33: torch.Tensor | None = None
+    whd: torch.Tensor | None = None
+
+    FIELDS = ("xm", "kx", "gradh", "prho", "divv", "curlv", "alpha")
+    GRAD_V = ("dV11", "dV12", "dV13", "dV22", "dV23", "dV33")
+
+    @property
+    def dtype(self):
+        return torch.float64 if self.precision == FP64 else torch.float32
+
+    def allocate(self, device, grad_v=False, sinc_index=6.0):
+        for f in self.FIELDS + (self.GRAD_V if grad_v else ()):
+            if getattr(self, f) is None:
+                setattr(self, f, torch.zeros(self.n, dtype=self.dtype, device=device))
+        if self.alpha is not None and not bool(self.alpha.any()):
+            self.alpha.fill_(self.alphamin)
+        if self.whd is None:
+            w64, w32 = kernel_derivative_tables(sinc_index)
+            self.whd = torch.from_numpy(w64 if self.precision == FP64 else w32).to(device)
+        return self
+
+    def struct(self) -> VeFields:
+        v = VeFields()
+        for f in self.FIELDS + self.GRAD_V + ("whd",):
+            t = getattr(self, f)
+            setattr(v, f, t.data_ptr() if t is not None else None)
+        v.alphamin, v.alphamax, v.decay = self.alphamin, self.alphamax, self.decay
+        v.Atmin, v.Atmax, v.ramp = self.Atmin, self.Atmax, self.ramp
+        return v
+
+
+def _ve_call(ctx, fn, d: ParticlesData, ve: VeData, box: Box, groups, *extra):
+    s = d.struct()
+    v = ve.struct()
+    ctx.check(fn(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(v), C.byref(box), *extra))
+    return s
+
+
+def compute_xmass(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
+    _ve_call(ctx, ctx.lib.sphx_compute_xmass, d, ve, box, groups)
+
+
+def compute_ve_def_gradh(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
+    _ve_call(ctx, ctx.lib.sphx_compute_ve_def_gradh, d, ve, box, groups)
+
+
+def compute_eos_ve(ctx: Context, d: ParticlesData, ve: VeData, first=0, last=None):
+    s = d.struct()
+    v = ve.struct()
+    ctx.check(ctx.lib.sphx_compute_eos_ve(ctx.h, first, d.n if last is None else last, C.byref(s), C.byref(v)))
+
+
+def compute_iad_divv_curlv(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
+    _ve_call(ctx, ctx.lib.sphx_compute_iad_divv_curlv, d, ve, box, groups)
+
+
+def compute_av_switches(ctx: Context, d: ParticlesData, ve: VeData, box: Box, dt: float, groups=None):
+    _ve_call(ctx, ctx.lib.sphx_compute_av_switches, d, ve, box, groups, C.c_double(dt))
+
+
+def compute_momentum_energy_ve(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
+    s = _ve_call(ctx, ctx.lib.sphx_compute_momentum_energy_ve, d, ve, box, groups)
+    d.minDtCourant = s.minDtCourant
+    return d.minDtCourant
+
+
 # ------------------------------------------------------------------------------------------- adjoints
 def density_adjoint(ctx: Context, d: ParticlesData, box: Box, g_rho, groups=None, want_h=True):
     """Adjoint of compute_density on the context's lists: (dL/dx, dL/dy, dL/dz[, dL/dh]) for
@@ -608,6 +718,77 @@ class HydroStep:
             return {}
         self.events[-1].synchronize()
         return {s: self.events[k].elapsed_time(self.events[k + 1]) for k, s in enumerate(STAGES)}
+
+
+VE_STAGES = ("box", "keys", "sort", "gather", "octree", "link", "centers", "groups", "neighbors", "xmass",
+             "gradh", "eos", "iad_divv_curlv", "av_switches", "momentum_energy")
+
+
+class VeHydroStep(HydroStep):
+    """One single-rank force evaluation of the VE propagator (HydroVeProp::computeForces,
+    main/src/propagator/ve_hydro.hpp:126-205, avClean off): the same sync, tree, groups and
+    neighbor search as HydroStep, then xmass, veDefGradh, EOS, iadDivvCurlv, AVswitches (over
+    the time step ``dt``) and the VE momentum-energy loop.  ``self.ve`` holds the VE fields;
+    alpha is conserved: pass it in ``inputs`` (unsorted) or it starts at alphamin."""
+
+    def __init__(self, ctx: Context, n: int, box: Box, precision=FP64, grad_v=False, **kw):
+        super().__init__(ctx, n, box, precision, **kw)
+        self.ve = VeData(n, precision).allocate(ctx.device, grad_v=grad_v)
+
+    def compute_forces(self, inp: dict, timed=False, dt: float = 1e-6):
+        ctx, d, box = self.ctx, self.d, self.box
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(VE_STAGES) + 1)] if timed else None
+        mark = (lambda k: ev[k].record(ctx.stream)) if timed else (lambda k: None)
+        mark(0)
+        if not (box.bx and box.by and box.bz):
+            box = self.box = make_global_box(ctx, inp["x"], inp["y"], inp["z"], box)
+        mark(1)
+        self.keys.zero_()
+        compute_sfc_keys(ctx, inp["x"], inp["y"], inp["z"], box, self.keys)
+        mark(2)
+        ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
+        sort_by_key(ctx, self.keys, self.order)
+        mark(3)
+        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION:
+            if f in inp:
+                gather(ctx, self.order, inp[f], getattr(d, f))
+        if "alpha" in inp:
+            gather(ctx, self.order, inp["alpha"], self.ve.alpha)
+        mark(4)
+        leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
+        mark(5)
+        o = build_octree(ctx, leaves)
+        o.layout, o.counts = layout, counts
+        mark(6)
+        node_fp_centers(ctx, o, box)
+        self.octree = o
+        mark(7)
+        g = compute_groups(ctx, d.x, d.y, d.z, d.h, o, box) if self.spatial_groups else None
+        self.groups = g
+        mark(8)
+        find_neighbors_sfc(ctx, d, box, o, g)
+        mark(9)
+        ve = self.ve
+        compute_xmass(ctx, d, ve, box, g)
+        mark(10)
+        compute_ve_def_gradh(ctx, d, ve, box, g)
+        mark(11)
+        compute_eos_ve(ctx, d, ve)
+        mark(12)
+        compute_iad_divv_curlv(ctx, d, ve, box, g)
+        mark(13)
+        compute_av_switches(ctx, d, ve, box, dt, g)
+        mark(14)
+        compute_momentum_energy_ve(ctx, d, ve, box, g)
+        mark(15)
+        self.events = ev
+        return d
+
+    def stage_times_ms(self):
+        if not self.events:
+            return {}
+        self.events[-1].synchronize()
+        return {s: self.events[k].elapsed_time(self.events[k + 1]) for k, s in enumerate(VE_STAGES)}
 
 
 class Simulation:

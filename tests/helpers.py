@@ -69,3 +69,26 @@ def rel_err(a, b, floor=0.0):
     scale = np.maximum(np.abs(b), floor)
     scale[scale == 0] = 1.0
     return float(np.max(np.abs(a - b) / scale)) if a.size else 0.0
+
+
+def oracle_ve_step(inp, box, alpha, dt, ng0=100, ngmax=150, bucket=64, gamma=5.0 / 3.0, Kcour=0.2, grad_v=True):
+    """The reference VE stage order on the CPU (HydroVeProp::computeForces, avClean off):
+    returns a dict of SFC-sorted results."""
+    cpu = oracle.Cpu("oracle")
+    n = len(inp["x"])
+    keys = cpu.sfc_keys(inp["x"], inp["y"], inp["z"], box)
+    keys, order = cpu.sort_by_key(keys, np.arange(n, dtype=np.uint32))
+    s = {k: np.ascontiguousarray(v[order]) for k, v in inp.items()}
+    t = cpu.tree(keys, box, bucket)
+    wh, _, K = cpu.kernel_tables()
+    nb, nc, h = cpu.find_neighbors_sph(s["x"], s["y"], s["z"], s["h"], box, t, ng0, ngmax)
+    f = dict(s, h=h, wh=wh, whd=cpu.kernel_derivative_table(), alpha=np.ascontiguousarray(alpha[order]))
+    f["xm"] = cpu.ve_xmass(f, K, nb, nc, ngmax, box)
+    f["kx"], f["gradh"] = cpu.ve_def_gradh(f, K, nb, nc, ngmax, box)
+    e = cpu.ve_eos(f["u"], f["m"], f["kx"], f["xm"], f["gradh"], gamma)
+    f.update(prho=e["prho"], c=e["c"], rho=e["rho"], p=e["p"])
+    f.update(cpu.ve_iad_divv_curlv(f, K, nb, nc, ngmax, box, grad_v=grad_v))
+    f["alpha"] = cpu.ve_av_switches(f, K, nb, nc, ngmax, box, dt)
+    f["ax"], f["ay"], f["az"], f["du"], f["minDt"] = cpu.ve_momentum_energy(f, K, nb, nc, ngmax, box, Kcour=Kcour)
+    f.update(nc=nc, nbrs=nb, order=order, K=K)
+    return f

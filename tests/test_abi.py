@@ -42,6 +42,20 @@ def test_kernel_tables_match_oracle():
     assert np.array_equal(wh64, o64) and np.array_equal(wh32, o32) and K == oK
 
 
+def test_kernel_derivative_tables_match_oracle():
+    """The library's host table of the kernel derivative: the oracle's (which equals the
+    compiled reference's bit for bit) to 1e-14 relative; the sin/cos/pow calls of the two host
+    compilers round differently in a handful of the 20000 entries."""
+    from sphexa_amd.hydro import kernel_derivative_tables
+
+    w64, w32 = kernel_derivative_tables(6.0)
+    o64 = oracle.Cpu("oracle").kernel_derivative_table(6.0)
+    nz = o64 != 0
+    assert np.max(np.abs(w64[nz] - o64[nz]) / np.abs(o64[nz])) < 1e-14
+    assert np.array_equal(w64 == 0, o64 == 0)
+    assert np.max(np.abs(w32 - o64)) < 1e-6 * np.max(np.abs(o64))
+
+
 def test_octree_sizes():
     lib = _lib.load()
     ni, nn, npar = C.c_int32(), C.c_int32(), C.c_int32()
@@ -160,6 +174,7 @@ def test_target_groups_struct_points_into_bounds():
 
 
 @pytest.mark.parametrize("struct,cname", [("Particles", "sphx_particles"), ("Timestep", "sphx_timestep"),
+                                          ("VeFields", "sphx_ve_fields"),
                                           ("Box", "sphx_box"), ("Octree", "sphx_octree"), ("Groups", "sphx_groups")])
 def test_ctypes_struct_layout_matches_header(struct, cname):
     """Every field of the ctypes mirror sits at the offset the C compiler gives it."""

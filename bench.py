@@ -12,7 +12,8 @@ P(k) ~ k^(-11/3) for 1 <= |k| <= 3 at rms Mach 0.3, u = 1, h for ~100 neighbors.
 `value` is the mixed-precision step's particles/s; the FP64 step is timed in the same
 run and reported beside it.  Also reported: the momentum-energy adjoint with respect to
 positions after a step (lists recomputed), the integration step (time step, positions and
-energy, smoothing lengths) that completes a time step, the glass relaxation of a 100^3 jittered
+energy, smoothing lengths) that completes a time step, the VE propagator's force
+evaluation (the reference's default propagator) on the same input, the glass relaxation of a 100^3 jittered
 lattice with the density adjoint (200 iterations), the roofline of the dominant kernel
 (live HIP-event timing; HBM traffic from the committed PMC summary in profiles/), and
 the reference CPU path timed on the host cores on the same input (one step).
@@ -92,6 +93,38 @@ def time_me_adjoint(step, reps=3):
         step.momentum_energy_vjp(*g)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def run_ve(ctx, hydro, inp64, box, n, precision, steps, warmup):
+    """The VE propagator's force evaluation (HydroVeProp::computeForces, avClean off) on the
+    same input: sync, tree, groups, search, xmass, veDefGradh, EOS, iadDivvCurlv, AVswitches,
+    VE momentum-energy.  Returns ms per step, stage and kernel times."""
+    import torch
+
+    step = hydro.VeHydroStep(ctx, n, box, precision)
+    dt = step.d.dtype
+    inp = {k: (v if (k in ("x", "y", "z") or dt == torch.float64) else v.to(dt)) for k, v in inp64.items()}
+    for _ in range(warmup):
+        step.compute_forces(inp)
+    torch.cuda.synchronize()
+    step.compute_forces(inp, timed=True)
+    stages = step.stage_times_ms()
+    ctx.set_timing(True)
+    ctx.timing_summary()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.compute_forces(inp)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    kernels = ctx.timing_summary()
+    ctx.set_timing(False)
+    log(f"VE precision {precision}: {ms:.2f} ms/step over {steps} steps")
+    out = {"ms_per_step": ms, "particles_per_s": n / (ms * 1e-3), "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+           "kernels_ms_per_step": {k: round(v[1] / steps, 3) for k, v in kernels.items()}}
+    del step
+    torch.cuda.empty_cache()
+    return out
 
 
 def time_integrate(step, reps=5):
@@ -264,6 +297,8 @@ def main():
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one precision only (for profiling runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timings")
+    ap.add_argument("--no-ve", action="store_true", help="skip the VE propagator timings")
+    ap.add_argument("--ve-only", action="store_true", help="time only the VE propagator (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -289,7 +324,13 @@ def main():
     box = hydro.make_box(0.0, 1.0, BC_PERIODIC)
 
     results = {}
-    for prec in ([MIXED, FP64] if args.only is None else ([MIXED] if args.only == "mixed" else [FP64])):
+    precs = [MIXED, FP64] if args.only is None else ([MIXED] if args.only == "mixed" else [FP64])
+    if args.ve_only:
+        ve = {("mixed" if p == MIXED else "fp64"): run_ve(ctx, hydro, inp64, box, n, p, args.steps, args.warmup)
+              for p in precs}
+        print(json.dumps({"ve_hydro": ve}), flush=True)
+        return
+    for prec in precs:
         step, ms, stages, kernels, pairs, tests = run_precision(ctx, hydro, inp64, box, n, prec, args.steps,
                                                                 args.warmup)
         results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs,
@@ -346,6 +387,13 @@ def main():
     if not args.no_adjoint:
         log("glass relaxation (density adjoint), 100^3")
         out["density_adjoint_relaxation"] = glass_relaxation(ctx)
+    if not args.no_ve:
+        ve_steps = max(1, min(args.steps, 5))
+        out["ve_hydro"] = {("mixed" if p == MIXED else "fp64"): run_ve(ctx, hydro, inp64, box, n, p, ve_steps, 1)
+                           for p in precs}
+        out["ve_hydro"]["note"] = ("VE propagator force evaluation (xmass, veDefGradh, EOS, iadDivvCurlv, "
+                                   "AVswitches, VE momentum-energy; avClean off) on the same input, "
+                                   f"{ve_steps} timed steps")
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")

@@ -326,4 +326,89 @@ bool updateSmoothingLength(const GroupView& groups, Dataset& d)
     return removed != 0;
 }
 
+// ---- VE hydro (sph/include/sph/hydro_ve): the dataset also carries xm, kx, gradh, prho,
+// divv, curlv, alpha, whd and the VE constants; dV11..dV33 are used when they hold n values
+template<class Dataset>
+sphx_ve_fields toVeFields(Dataset& d)
+{
+    sphx_ve_fields v{};
+    v.xm = rawPtr(d.xm), v.kx = rawPtr(d.kx), v.gradh = rawPtr(d.gradh), v.prho = rawPtr(d.prho);
+    v.divv = rawPtr(d.divv), v.alpha = rawPtr(d.alpha), v.whd = rawPtr(d.whd);
+    v.curlv = d.curlv.size() == d.x.size() ? rawPtr(d.curlv) : nullptr;
+    if (d.dV11.size() == d.x.size())
+        v.dV11 = rawPtr(d.dV11), v.dV12 = rawPtr(d.dV12), v.dV13 = rawPtr(d.dV13), v.dV22 = rawPtr(d.dV22),
+        v.dV23 = rawPtr(d.dV23), v.dV33 = rawPtr(d.dV33);
+    v.alphamin = double(d.alphamin), v.alphamax = double(d.alphamax), v.decay = double(d.decay_constant);
+    v.Atmin = double(d.Atmin), v.Atmax = double(d.Atmax), v.ramp = double(d.ramp);
+    return v;
+}
+
+// sph::computeXMass (hydro_ve/xmass.hpp:69-75)
+template<class GroupView, class Dataset, class Box>
+void computeXMass(const GroupView& groups, Dataset& d, const Box& box)
+{
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    sphx_groups    g = toGroups(groups);
+    sphx_box       b = toBox(box);
+    check(sphx_compute_xmass(context(), &g, &p, &v, &b));
+}
+
+// sph::computeVeDefGradh (hydro_ve/ve_def_gradh.hpp:80-86)
+template<class GroupView, class Dataset, class Box>
+void computeVeDefGradh(const GroupView& groups, Dataset& d, const Box& box)
+{
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    sphx_groups    g = toGroups(groups);
+    sphx_box       b = toBox(box);
+    check(sphx_compute_ve_def_gradh(context(), &g, &p, &v, &b));
+}
+
+// sph::computeEOS, ideal gas with u (hydro_ve/eos.hpp:192-197)
+template<class Dataset>
+void computeEOS(size_t startIndex, size_t endIndex, Dataset& d)
+{
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    check(sphx_compute_eos_ve(context(), uint32_t(startIndex), uint32_t(endIndex), &p, &v));
+}
+
+// sph::computeIadDivvCurlv (hydro_ve/iad_divv_curlv.hpp:92-98)
+template<class GroupView, class Dataset, class Box>
+void computeIadDivvCurlv(const GroupView& groups, Dataset& d, const Box& box)
+{
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    sphx_groups    g = toGroups(groups);
+    sphx_box       b = toBox(box);
+    check(sphx_compute_iad_divv_curlv(context(), &g, &p, &v, &b));
+}
+
+// sph::computeAVswitches (hydro_ve/av_switches.hpp:81-87), over the step d.minDt
+template<class GroupView, class Dataset, class Box>
+void computeAVswitches(const GroupView& groups, Dataset& d, const Box& box)
+{
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    sphx_groups    g = toGroups(groups);
+    sphx_box       b = toBox(box);
+    check(sphx_compute_av_switches(context(), &g, &p, &v, &b, double(d.minDt)));
+}
+
+// sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:103-114); sets
+// d.minDtCourant.  AV cleaning and the per-group time steps (groupDt) are not supported.
+template<bool avClean, class GroupView, class Dataset, class Box>
+void computeMomentumEnergy(const GroupView& groups, float* groupDt, Dataset& d, const Box& box)
+{
+    static_assert(!avClean, "sphx_compat::computeMomentumEnergy: AV cleaning is not supported");
+    if (groupDt) throw std::runtime_error("sphx_compat::computeMomentumEnergy: groupDt is not supported");
+    sphx_particles p = toParticles(d);
+    sphx_ve_fields v = toVeFields(d);
+    sphx_groups    g = toGroups(groups);
+    sphx_box       b = toBox(box);
+    check(sphx_compute_momentum_energy_ve(context(), &g, &p, &v, &b));
+    d.minDtCourant = p.minDtCourant;
+}
+
 } // namespace sphx_compat

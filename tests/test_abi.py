@@ -104,6 +104,8 @@ struct Data {
     DevVec<unsigned> nc; unsigned ng0 = 100, ngmax = 150; double K = 0.79, gamma = 5.0 / 3.0, minDtCourant = 0; float Kcour = 0.2f;
     DevVec<float> x_m1, y_m1, z_m1, du_m1; DevVec<uint64_t> keys;
     double minDt = 1e-12, minDt_m1 = 1e-12, ttot = 0, minDtRho = 1e30, g = 0, etaAcc = 0.2;
+    DevVec<float> xm, kx, gradh, prho, divv, curlv, alpha, whd, dV11, dV12, dV13, dV22, dV23, dV33;
+    float alphamin = 0.05f, alphamax = 1.0f, decay_constant = 0.2f, Atmin = 0.1f, Atmax = 0.2f, ramp = 10.0f;
 };
 struct Group { unsigned firstBody, lastBody, numGroups; const unsigned* groupStart; const unsigned* groupEnd; };
 void use(Data& d, const Group& g) {
@@ -115,6 +117,12 @@ void use(Data& d, const Group& g) {
     sphx_compat::computeTimestep(0, 10, d);
     sphx_compat::computePositions(g, d, box, float(d.minDt), std::array<float, 1>{float(d.minDt_m1)});
     (void)sphx_compat::updateSmoothingLength(g, d);
+    sphx_compat::computeXMass(g, d, box);
+    sphx_compat::computeVeDefGradh(g, d, box);
+    sphx_compat::computeEOS(0, 10, d);
+    sphx_compat::computeIadDivvCurlv(g, d, box);
+    sphx_compat::computeAVswitches(g, d, box);
+    sphx_compat::computeMomentumEnergy<false>(g, nullptr, d, box);
     sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
     (void)sphx_compat::computeGroupSplits(0u, 0u, nullptr, nullptr, nullptr, (const double*)nullptr, nullptr, 0,
                                           nullptr, box);

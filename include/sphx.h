@@ -327,6 +327,8 @@ typedef struct sphx_ve_fields
     const void* whd;                          /* kernel-derivative table, SPHX_TABLE_SIZE entries */
     double      alphamin, alphamax, decay;    /* AV switches (particles_data.hpp:127-129)         */
     double      Atmin, Atmax, ramp;           /* Atwood ramp (particles_data.hpp:134-136)         */
+    int32_t     avClean;                      /* 1: AV cleaning in momentum-energy (needs dV)     */
+    int32_t     reserved;
 } sphx_ve_fields;
 
 /* Host-side derivative table: tabulateFunction of getSphKernelDerivative(sinc_n)
@@ -361,11 +363,17 @@ int sphx_compute_iad_divv_curlv(sphx_ctx* ctx, const sphx_groups* groups, sphx_p
 int sphx_compute_av_switches(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, sphx_ve_fields* ve,
                              const sphx_box* box, double dt);
 
-/* sph::computeMomentumEnergy<avClean = false> (hydro_ve/momentum_energy.hpp:40-117,
- * momentumAndEnergyJLoop momentum_energy_kern.hpp:65-221, tdpdTrho absent): ax, ay, az,
+/* sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:40-117,
+ * momentumAndEnergyJLoop momentum_energy_kern.hpp:65-221, tdpdTrho absent), avClean =
+ * ve->avClean (then the AV correction avRvCorrection :43-63 reads dV11..dV33): ax, ay, az,
  * du and d->minDtCourant.  Synchronises the stream once to return minDtCourant. */
 int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                     sphx_ve_fields* ve, const sphx_box* box);
+
+/* sph::rhoTimestep (sph/include/sph/ts_global.hpp:73-98): *minDtRho = Krho / |max divv|
+ * over [first, last) (ve->divv, field type of d->precision).  Synchronises the stream once. */
+int sphx_rho_timestep(sphx_ctx* ctx, uint32_t first, uint32_t last, const sphx_particles* d,
+                      const sphx_ve_fields* ve, double Krho, double* minDtRho);
 
 /* ---- adjoints -------------------------------------------------------------- */
 /* No counterpart in the reference (its loops are forward-only).  Vector-Jacobian

@@ -396,15 +396,26 @@ void computeAVswitches(const GroupView& groups, Dataset& d, const Box& box)
     check(sphx_compute_av_switches(context(), &g, &p, &v, &b, double(d.minDt)));
 }
 
+// sph::rhoTimestep (sph/include/sph/ts_global.hpp:73-98): d.Krho / |max divv|
+template<class Dataset>
+auto rhoTimestep(size_t first, size_t last, const Dataset& d)
+{
+    sphx_particles p = toParticles(const_cast<Dataset&>(d));
+    sphx_ve_fields v = toVeFields(const_cast<Dataset&>(d));
+    double         r = 0;
+    check(sphx_rho_timestep(context(), uint32_t(first), uint32_t(last), &p, &v, double(d.Krho), &r));
+    return r;
+}
+
 // sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:103-114); sets
-// d.minDtCourant.  AV cleaning and the per-group time steps (groupDt) are not supported.
+// d.minDtCourant.  The per-group time steps (groupDt) are not supported.
 template<bool avClean, class GroupView, class Dataset, class Box>
 void computeMomentumEnergy(const GroupView& groups, float* groupDt, Dataset& d, const Box& box)
 {
-    static_assert(!avClean, "sphx_compat::computeMomentumEnergy: AV cleaning is not supported");
     if (groupDt) throw std::runtime_error("sphx_compat::computeMomentumEnergy: groupDt is not supported");
     sphx_particles p = toParticles(d);
     sphx_ve_fields v = toVeFields(d);
+    v.avClean        = avClean ? 1 : 0;
     sphx_groups    g = toGroups(groups);
     sphx_box       b = toBox(box);
     check(sphx_compute_momentum_energy_ve(context(), &g, &p, &v, &b));

@@ -61,7 +61,8 @@ class Particles(C.Structure):
 class VeFields(C.Structure):
     _fields_ = [(nm, _V) for nm in ("xm", "kx", "gradh", "prho", "divv", "curlv", "alpha", "dV11", "dV12", "dV13",
                                     "dV22", "dV23", "dV33", "whd")] + \
-               [(nm, _D) for nm in ("alphamin", "alphamax", "decay", "Atmin", "Atmax", "ramp")]
+               [(nm, _D) for nm in ("alphamin", "alphamax", "decay", "Atmin", "Atmax", "ramp")] + \
+               [("avClean", _I32), ("reserved", _I32)]
 
 
 class Timestep(C.Structure):
@@ -121,6 +122,8 @@ SIGNATURES = {
                                            C.POINTER(Box), _D]),
     "sphx_compute_momentum_energy_ve": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles),
                                                   C.POINTER(VeFields), C.POINTER(Box)]),
+    "sphx_rho_timestep": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles), C.POINTER(VeFields), _D,
+                                    C.POINTER(_D)]),
     "sphx_make_global_box": (C.c_int, [_V, _V, _V, _V, _U64, C.POINTER(Box), C.POINTER(Box)]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "sphx_internal.hpp"
@@ -381,6 +382,8 @@ int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sp
     if (!d->vx || !d->vy || !d->vz || !d->c || !d->c11 || !d->c12 || !d->c13 || !d->c22 || !d->c23 || !d->c33 ||
         !d->ax || !d->ay || !d->az || !d->du || !ve->xm || !ve->kx || !ve->prho || !ve->alpha)
         return argError(ctx, "compute_momentum_energy_ve: null field");
+    if (ve->avClean && (!ve->dV11 || !ve->dV12 || !ve->dV13 || !ve->dV22 || !ve->dV23 || !ve->dV33))
+        return argError(ctx, "compute_momentum_energy_ve: AV cleaning needs dV11..dV33");
     if (!validBox(box)) return argError(ctx, "compute_momentum_energy_ve: invalid box");
     GroupsDev G;
     SphLists  lists;
@@ -401,6 +404,21 @@ int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sp
     std::memcpy(&v, &bits, sizeof(v));
     d->minDtCourant = v;
     return SPHX_OK;
+}
+
+int sphx_rho_timestep(sphx_ctx* ctx, uint32_t first, uint32_t last, const sphx_particles* d,
+                      const sphx_ve_fields* ve, double Krho, double* minDtRho)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!d || !ve || !ve->divv || !minDtRho) return argError(ctx, "rho_timestep: null argument");
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED) return argError(ctx, "rho_timestep: unknown precision");
+    if (last < first || last > d->n) return argError(ctx, "rho_timestep: bad range");
+    if (last == first)
+    {
+        *minDtRho = Krho / INFINITY; // max over nothing is -inf
+        return SPHX_OK;
+    }
+    return fast::launchRhoTimestep(ctx, first, last, d, ve, Krho, minDtRho);
 }
 
 int sphx_density_adjoint(sphx_ctx* ctx, const sphx_groups* groups, const sphx_particles* d, const sphx_box* box,

@@ -1,5 +1,10 @@
 // sphx_hydro_exact.hip - the std-SPH loops compiled with -ffp-contract=off
 // (FP64 results round operation by operation like the reference CPU loops).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
 #include "sphx_internal.hpp"
 
 #define SPHX_HYDRO_NS exact

@@ -511,6 +511,7 @@ class VeData:
     Atmin: float = 0.1
     Atmax: float = 0.2
     ramp: float = 1.0 / (0.2 - 0.1)
+    av_clean: bool = False
     xm: torch.Tensor | None = None
     kx: torch.Tensor | None = None
     gradh: torch.Tensor | None = None
@@ -551,6 +552,7 @@ class VeData:
             setattr(v, f, t.data_ptr() if t is not None else None)
         v.alphamin, v.alphamax, v.decay = self.alphamin, self.alphamax, self.decay
         v.Atmin, v.Atmax, v.ramp = self.Atmin, self.Atmax, self.ramp
+        v.avClean = int(bool(self.av_clean))
         return v
 
 
@@ -581,6 +583,16 @@ def compute_iad_divv_curlv(ctx: Context, d: ParticlesData, ve: VeData, box: Box,
 
 def compute_av_switches(ctx: Context, d: ParticlesData, ve: VeData, box: Box, dt: float, groups=None):
     _ve_call(ctx, ctx.lib.sphx_compute_av_switches, d, ve, box, groups, C.c_double(dt))
+
+
+def rho_timestep(ctx: Context, d: ParticlesData, ve: VeData, Krho: float = 0.06, first=0, last=None) -> float:
+    """sph::rhoTimestep (sph/include/sph/ts_global.hpp:73-98): Krho / |max divv|."""
+    s = d.struct()
+    v = ve.struct()
+    out = C.c_double()
+    ctx.check(ctx.lib.sphx_rho_timestep(ctx.h, first, d.n if last is None else last, C.byref(s), C.byref(v), Krho,
+                                        C.byref(out)))
+    return out.value
 
 
 def compute_momentum_energy_ve(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
@@ -726,14 +738,16 @@ VE_STAGES = ("box", "keys", "sort", "gather", "octree", "link", "centers", "grou
 
 class VeHydroStep(HydroStep):
     """One single-rank force evaluation of the VE propagator (HydroVeProp::computeForces,
-    main/src/propagator/ve_hydro.hpp:126-205, avClean off): the same sync, tree, groups and
-    neighbor search as HydroStep, then xmass, veDefGradh, EOS, iadDivvCurlv, AVswitches (over
-    the time step ``dt``) and the VE momentum-energy loop.  ``self.ve`` holds the VE fields;
+    main/src/propagator/ve_hydro.hpp:126-205): the same sync, tree, groups and neighbor
+    search as HydroStep, then xmass, veDefGradh, EOS, iadDivvCurlv with rhoTimestep
+    (``self.minDtRho``), AVswitches (over the time step ``dt``) and the VE momentum-energy
+    loop (with AV cleaning when ``av_clean``).  ``self.ve`` holds the VE fields;
     alpha is conserved: pass it in ``inputs`` (unsorted) or it starts at alphamin."""
 
-    def __init__(self, ctx: Context, n: int, box: Box, precision=FP64, grad_v=False, **kw):
+    def __init__(self, ctx: Context, n: int, box: Box, precision=FP64, grad_v=False, av_clean=False, **kw):
         super().__init__(ctx, n, box, precision, **kw)
-        self.ve = VeData(n, precision).allocate(ctx.device, grad_v=grad_v)
+        self.ve = VeData(n, precision, av_clean=av_clean).allocate(ctx.device, grad_v=grad_v or av_clean)
+        self.minDtRho = math.inf
 
     def compute_forces(self, inp: dict, timed=False, dt: float = 1e-6):
         ctx, d, box = self.ctx, self.d, self.box
@@ -776,6 +790,7 @@ class VeHydroStep(HydroStep):
         compute_eos_ve(ctx, d, ve)
         mark(12)
         compute_iad_divv_curlv(ctx, d, ve, box, g)
+        self.minDtRho = rho_timestep(ctx, d, ve)
         mark(13)
         compute_av_switches(ctx, d, ve, box, dt, g)
         mark(14)

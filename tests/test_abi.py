@@ -106,6 +106,7 @@ struct Data {
     double minDt = 1e-12, minDt_m1 = 1e-12, ttot = 0, minDtRho = 1e30, g = 0, etaAcc = 0.2;
     DevVec<float> xm, kx, gradh, prho, divv, curlv, alpha, whd, dV11, dV12, dV13, dV22, dV23, dV33;
     float alphamin = 0.05f, alphamax = 1.0f, decay_constant = 0.2f, Atmin = 0.1f, Atmax = 0.2f, ramp = 10.0f;
+    double Krho = 0.06;
 };
 struct Group { unsigned firstBody, lastBody, numGroups; const unsigned* groupStart; const unsigned* groupEnd; };
 void use(Data& d, const Group& g) {
@@ -123,6 +124,8 @@ void use(Data& d, const Group& g) {
     sphx_compat::computeIadDivvCurlv(g, d, box);
     sphx_compat::computeAVswitches(g, d, box);
     sphx_compat::computeMomentumEnergy<false>(g, nullptr, d, box);
+    sphx_compat::computeMomentumEnergy<true>(g, nullptr, d, box);
+    (void)sphx_compat::rhoTimestep(0, 10, d);
     sphx_compat::computeSfcKeys(nullptr, nullptr, nullptr, nullptr, 0, box);
     (void)sphx_compat::computeGroupSplits(0u, 0u, nullptr, nullptr, nullptr, (const double*)nullptr, nullptr, 0,
                                           nullptr, box);

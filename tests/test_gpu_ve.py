@@ -53,11 +53,11 @@ def cases():
     yield "sedov", inp, box, np.full(len(inp["x"]), 0.5)
 
 
-def run_gpu(ctx, inp, box, alpha, precision, dt):
+def run_gpu(ctx, inp, box, alpha, precision, dt, av_clean=False):
     from sphexa_amd import hydro
 
     n = len(inp["x"])
-    step = hydro.VeHydroStep(ctx, n, hbox(box), precision, grad_v=True)
+    step = hydro.VeHydroStep(ctx, n, hbox(box), precision, grad_v=True, av_clean=av_clean)
     if precision == 0:
         # the derivative table is an input of the loops: both sides read the oracle's, whose
         # host libm calls round like the reference's (the library's own table agrees to 1e-14)
@@ -75,6 +75,7 @@ def run_gpu(ctx, inp, box, alpha, precision, dt):
         out[k] = getattr(d, k).double().cpu().numpy()
     out["nc"] = d.nc.cpu().numpy().view(np.uint32)
     out["minDt"] = d.minDtCourant
+    out["minDtRho"] = step.minDtRho
     return out
 
 
@@ -153,3 +154,29 @@ def test_ve_mixed(ctx, case):
     for k in ("divv", "curlv", "dV11", "dV12", "dV13", "dV22", "dV23", "dV33", "alpha", "ax", "ay", "az", "du"):
         assert sc[k] <= 2e-3, (k, sc[k])
     assert got["minDt"] == pytest.approx(want["minDt"], rel=1e-4)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ve_av_cleaning_and_rho_timestep(ctx, precision):
+    """momentumAndEnergyJLoop<avClean = true> against the oracle's (pinned to the reference's
+    known answer and bit-exact to its compiled template); rhoTimestep = 0.06 / |max divv|.
+    FP64: 1e-10 of the largest magnitude (device exp in the correction); MIXED: 2e-3."""
+    name, inp, box, alpha = next(c for c in cases() if c[0] == "uniform")
+    print("case ve av-clean precision", precision, flush=True)
+    if precision == 1:
+        inp = {k: (v if k in ("x", "y", "z") else v.astype(np.float32).astype(np.float64)) for k, v in inp.items()}
+        alpha = alpha.astype(np.float32).astype(np.float64)
+    want = oracle_ve_step(inp, box, alpha, dt=1e-4)
+    wa = CPU.ve_momentum_energy({k: want[k] for k in want if isinstance(want[k], np.ndarray)}, want["K"],
+                                want["nbrs"], want["nc"], 150, box, av_clean=True)
+    got = run_gpu(ctx, inp, box, alpha, precision, 1e-4, av_clean=True)
+    tol = 1e-10 if precision == 0 else 2e-3
+    scale = max(np.max(np.abs(wa[k])) for k in range(3))
+    for k, nm in enumerate(("ax", "ay", "az")):
+        assert scale_err(got[nm], wa[k], scale) <= tol, nm
+    assert scale_err(got["du"], wa[3]) <= tol
+    assert got["minDt"] == pytest.approx(wa[4], rel=1e-10 if precision == 0 else 1e-4)
+    # the correction changes the result: not the plain loop's
+    assert np.max(np.abs(wa[0] - want["ax"])) > 1e-5 * scale
+    want_rho = 0.06 / abs(np.max(got["divv"]))
+    assert got["minDtRho"] == pytest.approx(want_rho, rel=1e-15)

@@ -122,9 +122,23 @@ def run_ve(ctx, hydro, inp64, box, n, precision, steps, warmup):
     log(f"VE precision {precision}: {ms:.2f} ms/step over {steps} steps")
     out = {"ms_per_step": ms, "particles_per_s": n / (ms * 1e-3), "stages_ms": {k: round(v, 3) for k, v in stages.items()},
            "kernels_ms_per_step": {k: round(v[1] / steps, 3) for k, v in kernels.items()}}
+    rho = step.d.rho.double().clone()
     del step
     torch.cuda.empty_cache()
-    return out
+    return out, rho
+
+
+def run_ve_both(ctx, hydro, inp64, box, n, precs, steps, warmup):
+    """run_ve for each precision; with both, the mixed VE density's error against FP64."""
+    from sphexa_amd._lib import MIXED
+
+    res, rho = {}, {}
+    for p in precs:
+        key = "mixed" if p == MIXED else "fp64"
+        res[key], rho[key] = run_ve(ctx, hydro, inp64, box, n, p, steps, warmup)
+    if len(rho) == 2:
+        res["rho_max_rel_err_mixed_vs_fp64"] = float(((rho["mixed"] - rho["fp64"]).abs() / rho["fp64"].abs()).max())
+    return res
 
 
 def time_integrate(step, reps=5):
@@ -326,8 +340,7 @@ def main():
     results = {}
     precs = [MIXED, FP64] if args.only is None else ([MIXED] if args.only == "mixed" else [FP64])
     if args.ve_only:
-        ve = {("mixed" if p == MIXED else "fp64"): run_ve(ctx, hydro, inp64, box, n, p, args.steps, args.warmup)
-              for p in precs}
+        ve = run_ve_both(ctx, hydro, inp64, box, n, precs, args.steps, args.warmup)
         print(json.dumps({"ve_hydro": ve}), flush=True)
         return
     for prec in precs:
@@ -389,8 +402,7 @@ def main():
         out["density_adjoint_relaxation"] = glass_relaxation(ctx)
     if not args.no_ve:
         ve_steps = max(1, min(args.steps, 5))
-        out["ve_hydro"] = {("mixed" if p == MIXED else "fp64"): run_ve(ctx, hydro, inp64, box, n, p, ve_steps, 1)
-                           for p in precs}
+        out["ve_hydro"] = run_ve_both(ctx, hydro, inp64, box, n, precs, ve_steps, 1)
         out["ve_hydro"]["note"] = ("VE propagator force evaluation (xmass, veDefGradh, EOS, iadDivvCurlv, "
                                    "AVswitches, VE momentum-energy; avClean off) on the same input, "
                                    f"{ve_steps} timed steps")

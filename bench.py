@@ -409,6 +409,12 @@ def main():
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")
+    if out["roofline"]["traffic"]:
+        # the same kernel against the HBM roofline: measured bytes per launch / launch time
+        gbs = out["roofline"]["traffic"] / (out["roofline"]["per_launch_ms"] * 1e-3) / 1e9
+        out["roofline"]["hbm"] = {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                                  "note": "FETCH_SIZE + WRITE_SIZE per launch (profiles/traffic_*.txt) over the "
+                                          "live launch time: the kernel is VALU-bound, not HBM-bound"}
     if not args.no_cpu:
         log("timing the CPU baseline")
         cs = args.cpu_side or args.side

@@ -144,9 +144,9 @@ def test_kernel_derivative_table():
     assert np.max(np.abs(fd - whd[1:-1])) < 2e-7
 
 
-def ve_case(n=3000, seed=31):
-    """A periodic uniform case whose VE fields come from the oracle's own loops in order."""
-    inp, box = uniform_box(n, seed=seed)
+def ve_case(n=3000, seed=31, periodic=True):
+    """A uniform case whose VE fields come from the oracle's own loops in order."""
+    inp, box = uniform_box(n, seed=seed, periodic=periodic)
     cpu = CPU
     keys = cpu.sfc_keys(inp["x"], inp["y"], inp["z"], box)
     keys, order = cpu.sort_by_key(keys, np.arange(n, dtype=np.uint32))
@@ -176,8 +176,9 @@ def run_ve(cpu, f, K, nb, nc, box):
 
 
 @need_ref
-def test_ve_loops_equal_reference():
-    f, K, nb, nc, box = ve_case()
+@pytest.mark.parametrize("periodic", [True, False])
+def test_ve_loops_equal_reference(periodic):
+    f, K, nb, nc, box = ve_case(periodic=periodic)
     a, ea, ma = run_ve(CPU, f, K, nb, nc, box)
     b, eb, mb = run_ve(REF, f, K, nb, nc, box)
     for k in ("xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "c33", "divv", "curlv", "dV11",

@@ -816,9 +816,17 @@ class Simulation:
     CONSERVED = ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION
 
     def __init__(self, ctx: Context, state: dict, box: Box, precision=FP64, ts: TimestepState | None = None,
-                 **step_kw):
+                 propagator: str = "std", **step_kw):
+        """propagator "std" (HydroProp) or "ve" (HydroVeProp: alpha is conserved too, and the
+        time step includes rhoTimestep).  Both carry the specific internal energy u (the VE
+        propagator's temperature is u / cv with a constant cv)."""
         n = state["x"].numel()
-        self.step_obj = HydroStep(ctx, n, box, precision, **step_kw)
+        if propagator not in ("std", "ve"):
+            raise ValueError(f"unknown propagator {propagator!r}")
+        self.ve = propagator == "ve"
+        self.step_obj = (VeHydroStep if self.ve else HydroStep)(ctx, n, box, precision, **step_kw)
+        if self.ve:
+            self.CONSERVED = Simulation.CONSERVED + ("alpha",)
         d = self.step_obj.d
         d.allocate(ctx.device, integration=True)
         dt = d.dtype
@@ -826,6 +834,8 @@ class Simulation:
         for f in self.CONSERVED:
             want = torch.float64 if f in ("x", "y", "z") else dt
             src = state.get(f)
+            if src is None and f == "alpha":
+                src = torch.full((n,), self.step_obj.ve.alphamin, dtype=want, device=ctx.device)
             self.state[f] = (torch.zeros(n, dtype=want, device=ctx.device) if src is None
                              else src.to(device=ctx.device, dtype=want).clone())
         self.ts = ts if ts is not None else TimestepState()
@@ -837,14 +847,20 @@ class Simulation:
 
     def step(self):
         st = self.step_obj
-        st.compute_forces(self.state)
+        if self.ve:
+            # AV switches over the current step (d.minDt, ve_hydro.hpp:172); rhoTimestep (:166) feeds
+            # computeTimestep
+            st.compute_forces(self.state, dt=self.ts.minDt)
+            self.ts.minDtRho = st.minDtRho
+        else:
+            st.compute_forces(self.state)
         st.integrate(self.ts)
         # the integrated SFC-ordered fields become the state; the old state buffers receive the
         # next step's permutation
-        d = st.d
         for f in self.CONSERVED:
-            cur = getattr(d, f)
-            setattr(d, f, self.state[f])
+            holder = st.ve if f == "alpha" else st.d
+            cur = getattr(holder, f)
+            setattr(holder, f, self.state[f])
             self.state[f] = cur
         self.steps += 1
         return self.ts

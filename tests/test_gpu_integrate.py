@@ -230,3 +230,64 @@ def test_sedov_time_steps_fp64_exact(ctx):
     for k in ("vx", "vy", "vz"):
         assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * np.max(np.abs(want[k])), k
     assert np.max(np.abs(want["vx"])) > 0  # the blast has started to move particles
+
+
+def oracle_ve_time_steps(inp, box, steps, ts, alpha):
+    """The reference's VE propagator on the CPU: sync (sort), computeForces (AV switches over
+    the current minDt, rhoTimestep), integrate."""
+    from tests.helpers import oracle_ve_step
+
+    st = {k: v.copy() for k, v in inp.items()}
+    st["alpha"] = alpha.copy()
+    for _ in range(steps):
+        a = st.pop("alpha")
+        r = oracle_ve_step(st, box, a, dt=ts[0])
+        s = {k: np.ascontiguousarray(v[r["order"]]) for k, v in st.items()}
+        s.update(h=r["h"], ax=r["ax"], ay=r["ay"], az=r["az"], du=r["du"])
+        min_dt_rho = 0.06 / abs(np.max(r["divv"]))
+        t, _ = CPU.compute_timestep(ts, min_dt_courant=r["minDt"], min_dt_rho=min_dt_rho)
+        ts = list(t)
+        s = CPU.compute_positions(s, ts[0], ts[1], box)
+        s["h"], _, removed = CPU.update_smoothing_length(100, r["nc"], s["h"])
+        assert not removed
+        st = {k: s[k] for k in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u", "x_m1", "y_m1", "z_m1", "du_m1")}
+        st["alpha"] = r["alpha"]
+    return st, ts
+
+
+def test_ve_time_steps_fp64_exact(ctx):
+    """Three VE time steps (HydroVeProp: computeForces + integrate) against the oracle's; the
+    loops are bit-identical in FP64-exact mode, h differs by device pow (1 ulp): 1e-12."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64
+
+    print("case ve sedov 16^3 three time steps", flush=True)
+    inp, box = sedov_lattice(16)
+    n = len(inp["x"])
+    inp["vx"] = 0.1 * np.sin(2 * np.pi * inp["x"])
+    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
+        inp[k] = np.zeros(n)
+    alpha = np.full(n, 0.5)
+    ts0 = [1e-6, 1e-6, 0.0]
+    want, ts_want = oracle_ve_time_steps(inp, box, 3, ts0, alpha)
+
+    ctx.set_fp64_exact(True)
+    try:
+        state = {k: torch.from_numpy(v).to("cuda") for k, v in inp.items()}
+        state["alpha"] = torch.from_numpy(alpha).to("cuda")
+        sim = hydro.Simulation(ctx, state, hbox(box), FP64, propagator="ve",
+                               ts=hydro.TimestepState(minDt=ts0[0], minDt_m1=ts0[1], ttot=ts0[2]))
+        # the derivative table is an input: the oracle's, as in the VE parity tests
+        sim.step_obj.ve.whd = torch.from_numpy(CPU.kernel_derivative_table()).to("cuda")
+        for _ in range(3):
+            sim.step()
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_fp64_exact(False)
+    assert sim.ts.minDt == pytest.approx(ts_want[0], rel=1e-12)
+    assert sim.ts.ttot == pytest.approx(ts_want[2], rel=1e-12)
+    got = {k: v.double().cpu().numpy() for k, v in sim.state.items()}
+    for k in ("x", "y", "z", "h", "u", "x_m1", "du_m1", "alpha"):
+        assert rel_err(got[k], want[k], floor=1e-12 * np.max(np.abs(want[k]))) <= 1e-12, k
+    for k in ("vx", "vy", "vz"):
+        assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * np.max(np.abs(want[k])), k

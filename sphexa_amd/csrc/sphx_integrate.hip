@@ -235,6 +235,7 @@ int sphx_compute_positions(sphx_ctx* ctx, const sphx_groups* groups, sphx_partic
 {
     if (!ctx) return SPHX_ERR_ARG;
     if (!d || !ts) return argError(ctx, "compute_positions: null argument");
+    if (d->n == 0) return SPHX_OK; // nothing to move (empty fields may be null)
     if (!d->x || !d->y || !d->z || !d->x_m1 || !d->y_m1 || !d->z_m1 || !d->vx || !d->vy || !d->vz || !d->ax ||
         !d->ay || !d->az || !d->h || !d->u || !d->du || !d->du_m1)
         return argError(ctx, "compute_positions: null field");
@@ -272,6 +273,8 @@ int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_
                                  int* keysRemoved)
 {
     if (!ctx) return SPHX_ERR_ARG;
+    if (keysRemoved) *keysRemoved = 0;
+    if (d && d->n == 0) return SPHX_OK; // nothing to update (empty fields may be null)
     if (!d || !d->h || !d->nc) return argError(ctx, "update_smoothing_length: null field");
     GroupsDev G;
     SPHX_TRY(groupsFromArgs(ctx, groups, 0, uint32_t(d->n), G));

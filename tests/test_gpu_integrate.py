@@ -291,3 +291,26 @@ def test_ve_time_steps_fp64_exact(ctx):
         assert rel_err(got[k], want[k], floor=1e-12 * np.max(np.abs(want[k]))) <= 1e-12, k
     for k in ("vx", "vy", "vz"):
         assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * np.max(np.abs(want[k])), k
+
+
+def test_integration_edge_sizes(ctx):
+    """One particle and an empty range: the kernels handle ragged and empty launches."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64
+
+    print("case integration edge sizes", flush=True)
+    st = state(1, seed=25)
+    box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
+    want = CPU.compute_positions(st, 1e-3, 1e-3, box)
+    d = to_dataset(st, FP64)
+    hydro.compute_positions(ctx, d, hbox(box), hydro.TimestepState(minDt=1e-3, minDt_m1=1e-3))
+    got = from_dataset(d)
+    for k in FIELDS:
+        assert np.array_equal(got[k], want[k]), k
+    ts = hydro.TimestepState(minDt=1e-3, minDt_m1=1e-3, g=1.0)
+    hydro.compute_timestep(ctx, d, ts, first=0, last=0)  # empty range: no acceleration limit
+    assert ts.minDtAcc == np.inf and ts.minDt == 1.1e-3
+    e = hydro.ParticlesData(0, FP64)
+    e.h = torch.zeros(0, dtype=torch.float64, device="cuda")
+    e.nc = torch.zeros(0, dtype=torch.int32, device="cuda")
+    assert not hydro.update_smoothing_length(ctx, e)

@@ -56,6 +56,10 @@ def cases():
     inp, box = uniform_box(3000, seed=43, periodic=False, lo=-1.0, hi=1.5)
     box = CPU.make_global_box(inp["x"], inp["y"], inp["z"], box)
     yield "open", inp, box, np.random.default_rng(43).uniform(0.05, 1.0, len(inp["x"]))
+    # ragged: 70 particles, two groups (64 + 6), open box, everyone near everyone
+    inp, box = uniform_box(70, seed=44, periodic=False, ng=60)
+    box = CPU.make_global_box(inp["x"], inp["y"], inp["z"], box)
+    yield "tiny", inp, box, np.random.default_rng(44).uniform(0.05, 1.0, len(inp["x"]))
 
 
 def ng0_of(box):
@@ -109,7 +113,7 @@ def field_errors(got, want):
     return {k: scale_err(got[k], want[k], scale[k]) for k in FIELDS}
 
 
-@pytest.mark.parametrize("case", ["uniform", "sedov", "open"])
+@pytest.mark.parametrize("case", ["uniform", "sedov", "open", "tiny"])
 def test_ve_fp64_exact(ctx, case):
     name, inp, box, alpha = next(c for c in cases() if c[0] == case)
     print("case ve fp64-exact", name, flush=True)
@@ -127,7 +131,7 @@ def test_ve_fp64_exact(ctx, case):
     assert got["minDt"] == pytest.approx(want["minDt"], rel=1e-12)
 
 
-@pytest.mark.parametrize("case", ["uniform", "sedov", "open"])
+@pytest.mark.parametrize("case", ["uniform", "sedov", "open", "tiny"])
 def test_ve_fp64(ctx, case):
     name, inp, box, alpha = next(c for c in cases() if c[0] == case)
     print("case ve fp64", name, flush=True)
@@ -140,7 +144,7 @@ def test_ve_fp64(ctx, case):
     assert got["minDt"] == pytest.approx(want["minDt"], rel=1e-10)
 
 
-@pytest.mark.parametrize("case", ["uniform", "sedov", "open"])
+@pytest.mark.parametrize("case", ["uniform", "sedov", "open", "tiny"])
 def test_ve_mixed(ctx, case):
     name, inp, box, alpha = next(c for c in cases() if c[0] == case)
     print("case ve mixed", name, flush=True)

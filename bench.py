@@ -407,6 +407,8 @@ def main():
                                    "AVswitches, VE momentum-energy; avClean off) on the same input, "
                                    f"{ve_steps} timed steps")
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
+    # the device's sustained FMA rates, measured beside the published peaks
+    out["roofline"]["valu_measured_TFLOPs"] = {"fp32_scalar": ctx.valu_peak(MIXED), "fp64": ctx.valu_peak(FP64)}
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")
     if out["roofline"]["traffic"]:

@@ -379,12 +379,25 @@ def main():
         "stages_ms": {k: round(v, 3) for k, v in R["stages"].items()},
         "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in R["kernels"].items()},
     }
+    # whole-step algorithmic rates: pair flops of the three loops (19 + 37 + 121 per pair, 60 per
+    # particle for the IAD inverse), 11 flops per candidate test of the search, and the
+    # compulsory HBM bytes per particle of the step (keys, sort, permutation, loops)
+    def step_totals(res, prec):
+        flop = (19.0 + 37.0 + 121.0) * res["pairs"] + IAD_FLOP_PER_PARTICLE * n + FLOP_PER_TEST * (res["tests"] or 0)
+        nbytes = (532.0 if prec == MIXED else 716.0) * n
+        t = res["ms"] * 1e-3
+        return {"algorithmic_TFLOPs": flop / t / 1e12, "compulsory_GBs": nbytes / t / 1e9,
+                "hbm_frac": nbytes / t / 1e9 / PEAK_HBM_GBS,
+                "note": "step flops = 177 per pair + 60 per particle + 11 per candidate test; compulsory bytes "
+                        f"{'532' if prec == MIXED else '716'} per particle"}
+    out["step_totals"] = step_totals(R, main_prec)
     if FP64 in results and MIXED in results:
         F = results[FP64]
         out["fp64"] = {"value": n / (F["ms"] * 1e-3), "ms_per_step": F["ms"],
                        "stages_ms": {k: round(v, 3) for k, v in F["stages"].items()},
                        "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in F["kernels"].items()}}
         out["fp64"]["integrate_ms"] = F["integrate"]["ms"]
+        out["fp64"]["step_totals"] = step_totals(F, FP64)
         out["mixed_over_fp64"] = F["ms"] / R["ms"]
         rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
         out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())

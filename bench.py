@@ -49,10 +49,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup):
+def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup, **step_kw):
     import torch
 
-    step = hydro.HydroStep(ctx, n, box, precision)
+    step = hydro.HydroStep(ctx, n, box, precision, **step_kw)
     dt = step.d.dtype
     inp = {k: (v if (k in ("x", "y", "z") or dt == torch.float64) else v.to(dt)) for k, v in inp64.items()}
     for w in range(warmup):
@@ -299,6 +299,49 @@ def cpu_baseline(f, side, steps):
     return out
 
 
+def run_other_workload(args, ctx):
+    """The evrard and uniform800 configurations: the same step and timing, mixed and FP64, with
+    the neighbor-count distribution (a parity configuration, not the metric's)."""
+    import numpy as np
+    import torch
+
+    from sphexa_amd import hydro, workloads
+    from sphexa_amd._lib import BC_OPEN, BC_PERIODIC, FP64, MIXED
+
+    if args.workload == "evrard":
+        f, (lo, hi, per) = workloads.evrard(4_194_304, seed=105)
+        kw, desc = dict(ng0=100, ngmax=512), "Evrard-like 1/r sphere, 4,194,304 particles, open box, ~100 neighbors"
+    else:
+        f, (lo, hi, per) = workloads.uniform_random(1_000_000, seed=20261202)
+        kw, desc = dict(ng0=0, ngmax=1024), "uniform random 1,000,000 particles, periodic, h as written (~800 " \
+                                            "neighbors), fixed h"
+    n = len(f["x"])
+    box = hydro.make_box(lo, hi, BC_PERIODIC if per else BC_OPEN)
+    inp64 = {k: torch.tensor(v, dtype=torch.float64, device=ctx.device) for k, v in f.items()}
+    res = {}
+    for prec in (MIXED, FP64):
+        step, ms, stages, kernels, pairs, tests = run_precision(ctx, hydro, inp64, box, n, prec, args.steps,
+                                                                args.warmup, **kw)
+        nc = step.d.nc.to(torch.int64).cpu().numpy() - 1
+        res[prec] = dict(ms=ms, pairs=pairs, tests=tests, kernels={k: round(v[1] / args.steps, 3)
+                                                                 for k, v in kernels.items()},
+                         nc={"min": int(nc.min()), "median": float(np.median(nc)), "p99": float(np.percentile(nc, 99)),
+                             "max": int(nc.max())}, rho=step.d.rho.double().clone())
+        del step
+        torch.cuda.empty_cache()
+    R, F = res[MIXED], res[FP64]
+    out = {"metric": METRIC, "value": n / (R["ms"] * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": R["ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": desc, "particles": n, **kw, "parallelism": "single GPU"},
+           "pairs_per_s": R["pairs"] / (R["ms"] * 1e-3), "candidate_tests_per_pair": R["tests"] / max(1, R["pairs"]),
+           "neighbors": R["nc"], "kernels_ms_per_step": R["kernels"],
+           "fp64": {"value": n / (F["ms"] * 1e-3), "ms_per_step": F["ms"], "kernels_ms_per_step": F["kernels"]},
+           "mixed_over_fp64": F["ms"] / R["ms"],
+           "rho_max_rel_err_mixed_vs_fp64": float(((R["rho"] - F["rho"]).abs() / F["rho"].abs()).max())}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +356,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timings")
     ap.add_argument("--no-ve", action="store_true", help="skip the VE propagator timings")
     ap.add_argument("--ve-only", action="store_true", help="time only the VE propagator (profiling runs)")
+    ap.add_argument("--workload", choices=["turbulence", "evrard", "uniform800"], default="turbulence",
+                    help="turbulence: the metric's 256^3 box (default); evrard: 4M-particle 1/r sphere in an open "
+                         "box (ngmax 512); uniform800: 1M uniform random particles, h as written (~800 "
+                         "neighbors, fixed h, ngmax 1024)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -330,9 +377,12 @@ def main():
     from sphexa_amd._lib import BC_PERIODIC, FP64, MIXED
     from sphexa_amd.workloads import turbulence
 
+    ctx = hydro.Context(0)
+    if args.workload != "turbulence":
+        run_other_workload(args, ctx)
+        return
     n = args.side ** 3
     log(f"generating turbulence {args.side}^3 = {n} particles")
-    ctx = hydro.Context(0)
     inp64, _ = turbulence(args.side, device=ctx.device)
     torch.cuda.synchronize()
     box = hydro.make_box(0.0, 1.0, BC_PERIODIC)

@@ -367,8 +367,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              f"(the single-rank path has no multi-GPU decomposition).", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the single-rank hydro path has no multi-GPU decomposition).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -88,7 +88,7 @@ def test_density_gradient_vs_oracle_finite_differences():
         return ((rho - rho.mean()) ** 2).sum()
 
     probe = np.random.default_rng(3).choice(len(s["x"]), 24, replace=False)
-    assert _fd_check(lo, lt, s, probe, 2e-9) < 5e-6
+    assert _fd_check(lo, lt, s, probe, 2e-9) < 1e-6  # measured 2.2e-7
 
 
 def test_density_h_gradient_vs_oracle_finite_differences():
@@ -103,12 +103,15 @@ def test_density_h_gradient_vs_oracle_finite_differences():
         r = CPU.density(nb, nc, ngmax, s["x"], s["y"], s["z"], hh, s["m"], wh, K, box)
         return float(np.sum((r - r.mean()) ** 2))
 
+    worst = 0.0
     for p in np.random.default_rng(4).choice(len(h), 16, replace=False):
         hp, hm = s["h"].copy(), s["h"].copy()
-        hp[p] += 1e-10
-        hm[p] -= 1e-10
-        fd = (lo(hp) - lo(hm)) / 2e-10
-        assert abs(fd - float(gh[p])) / scale < 2e-5
+        hp[p] += 1e-9
+        hm[p] -= 1e-9
+        fd = (lo(hp) - lo(hm)) / 2e-9
+        worst = max(worst, abs(fd - float(gh[p])) / scale)
+    # step 1e-9: rounding dominates below it (1e-10: 2.6e-6, 1e-11: 3.2e-5); measured 3.0e-7
+    assert worst < 1e-6
 
 
 def test_momentum_energy_gradient_vs_oracle_finite_differences():
@@ -133,4 +136,4 @@ def test_momentum_energy_gradient_vs_oracle_finite_differences():
         return sum((R.tt(W[k]) * out[k]).sum() for k in range(4))
 
     probe = rng.choice(len(rho), 24, replace=False)
-    assert _fd_check(lo, lt, s, probe, 2e-9) < 5e-5
+    assert _fd_check(lo, lt, s, probe, 2e-9) < 1e-6  # measured 1.4e-8

@@ -318,6 +318,14 @@ int sphx_compute_positions(sphx_ctx* ctx, const sphx_groups* groups, sphx_partic
 int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, uint64_t* keys,
                                  int* keysRemoved);
 
+/* Host-side factors of updateH (sph/include/sph/kernels.hpp:27-33), which every h update
+ * of the library applies as h' = (h * 0.5) * f[nc]: f64[nc] = pow(1 + 1023 ng0 / nc, 0.1)
+ * in double and f32[nc] the same in float (powf), for nc in [0, count), computed with the
+ * host libm as the reference CPU path computes them.  The library tabulates nc < 131072
+ * per context; sphx_find_neighbors_sph and sphx_update_smoothing_length return an error
+ * for a count beyond that.  Either output may be NULL. */
+int sphx_smoothing_length_factors(uint32_t ng0, uint32_t count, double* f64, float* f32);
+
 /* ---- VE hydro (the reference's default propagator) ------------------------- */
 /* The fields of the volume-element (VE) loops that sphx_particles does not hold
  * (sph/include/sph/particles_data.hpp:218-246); every void* is double* for SPHX_FP64

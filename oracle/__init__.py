@@ -142,6 +142,7 @@ class Cpu:
         else:
             f("compute_positions", None, [_U64, _U, _U] + [_V] * 16 + [_D, _D, _V])
         f("update_smoothing_length", _I, [_U, _U, _U, _V, _V, _V])
+        f("update_smoothing_length_f32", _I, [_U, _U, _U, _V, _V, _V])
         pre = [] if kind == "oracle" else [_U64]
         loop = pre + [_U, _U, _V, _V, _U, _V, _V]
         f("ve_xmass", None, loop + [_V])
@@ -151,6 +152,7 @@ class Cpu:
         f("ve_av_switches", None, loop + [_D, _D, _D, _D, _V])
         f("ve_momentum_energy", _D, loop + [_I, _D, _D, _D, C.c_float, _V, _V, _V, _V])
         f("kernel_derivative_table", None, [_D, _V])
+        f("limit_box_shrinking", None, [_V, _V, _D, _V])
 
     def _f(self, name, res, args):
         fn = getattr(self.lib, f"{self.pre}_{name}")
@@ -247,6 +249,11 @@ class Cpu:
     def make_global_box(self, x, y, z, prev):
         out = Box()
         self._make_global_box(_p(x), _p(y), _p(z), len(x), C.byref(prev), C.byref(out))
+        return out
+
+    def limit_box_shrinking(self, fitting, previous, shrink_limit=0.05):
+        out = Box()
+        self._limit_box_shrinking(C.byref(fitting), C.byref(previous), shrink_limit, C.byref(out))
         return out
 
     # ---------------------------------------------------------------- neighbors
@@ -356,13 +363,15 @@ class Cpu:
         self._compute_positions(*self._n(n), first, last, *args, dt, dt_m1, C.byref(box))
         return o
 
-    def update_smoothing_length(self, ng0, nc, h, keys=None, first=0, last=None):
-        """updateSmoothingLengthCpu on copies: (h, keys or None, keysRemoved)."""
-        h = np.array(h, dtype=np.float64)
+    def update_smoothing_length(self, ng0, nc, h, keys=None, first=0, last=None, dtype=np.float64):
+        """updateSmoothingLengthCpu on copies: (h, keys or None, keysRemoved); dtype float32
+        runs the FP32 instantiation (the reference app's h type)."""
+        h = np.array(h, dtype=dtype)
         nc = np.ascontiguousarray(nc, np.uint32)
         k = None if keys is None else np.array(keys, dtype=np.uint64)
         last = len(h) if last is None else last
-        r = self._update_smoothing_length(first, last, ng0, _p(nc), _p(h), _p(k))
+        fn = self._update_smoothing_length_f32 if h.dtype == np.float32 else self._update_smoothing_length
+        r = fn(first, last, ng0, _p(nc), _p(h), _p(k))
         return h, k, bool(r)
 
 

@@ -434,6 +434,13 @@ int ref_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, con
     return sph::updateSmoothingLengthCpu(first, last, ng0, nc, h, keys) ? 1 : 0;
 }
 
+// the same with the reference app's FP32 h (HydroType = float, sph/types.hpp:39-46)
+int ref_update_smoothing_length_f32(uint32_t first, uint32_t last, unsigned ng0, const unsigned* nc, float* h,
+                                    uint64_t* keys)
+{
+    return sph::updateSmoothingLengthCpu(first, last, ng0, nc, h, keys) ? 1 : 0;
+}
+
 
 // ---- VE loops: the reference's computeXMassImpl, computeVeDefGradhImpl,
 // computeIdealGasEOS_Impl, computeIadDivvCurlvImpl, computeAVswitchesImpl and
@@ -523,6 +530,17 @@ double ref_ve_momentum_energy(uint64_t n, uint32_t first, uint32_t last, const u
     if (avClean) sph::computeMomentumEnergyImpl<true>(first, last, d, toBox(b));
     else sph::computeMomentumEnergyImpl<false>(first, last, d, toBox(b));
     return d.minDtCourant;
+}
+
+// cstone::limitBoxShrinking (domain/include/cstone/sfc/box.hpp:397-414)
+void ref_limit_box_shrinking(const sphx_box* fit, const sphx_box* prev, double shrinkLimit, sphx_box* out)
+{
+    auto r    = cstone::limitBoxShrinking(toBox(fit), toBox(prev), shrinkLimit);
+    *out      = *prev;
+    out->xmin = r.xmin(), out->xmax = r.xmax();
+    out->ymin = r.ymin(), out->ymax = r.ymax();
+    out->zmin = r.zmin(), out->zmax = r.zmax();
+    out->bx = int32_t(r.boundaryX()), out->by = int32_t(r.boundaryY()), out->bz = int32_t(r.boundaryZ());
 }
 
 void ref_kernel_derivative_table(double sincIndex, double* whd)

@@ -1033,6 +1033,22 @@ void orc_make_global_box(const double* x, const double* y, const double* z, uint
     }
 }
 
+// limitBoxShrinking (domain/include/cstone/sfc/box.hpp:397-414): each side of the fitted box
+// moves inward by at most shrinkLimit of the previous box's length; the boundary types are
+// the previous box's.  Domain::sync applies it from the second call on
+// (domain/assignment.hpp:104-106).
+void orc_limit_box_shrinking(const sphx_box* fit, const sphx_box* prev, double shrinkLimit, sphx_box* out)
+{
+    const double lx = prev->xmax - prev->xmin, ly = prev->ymax - prev->ymin, lz = prev->zmax - prev->zmin;
+    *out      = *prev;
+    out->xmin = std::min(fit->xmin, prev->xmin + shrinkLimit * lx);
+    out->xmax = std::max(fit->xmax, prev->xmax - shrinkLimit * lx);
+    out->ymin = std::min(fit->ymin, prev->ymin + shrinkLimit * ly);
+    out->ymax = std::max(fit->ymax, prev->ymax - shrinkLimit * ly);
+    out->zmin = std::min(fit->zmin, prev->zmin + shrinkLimit * lz);
+    out->zmax = std::max(fit->zmax, prev->zmax - shrinkLimit * lz);
+}
+
 
 // --------------------------------------------------------------------------- integration
 // accelerationTimestep, CPU branch (sph/include/sph/ts_global.hpp:48-71) without the
@@ -1151,6 +1167,31 @@ int orc_update_smoothing_length(uint32_t first, uint32_t last, unsigned ng0, con
             removed = 1;
         }
         h[i] = orc_update_h(ng0, nc[i], h[i]);
+    }
+    return removed;
+}
+
+// updateH<float> (sph/kernels.hpp:27-33): the reference app's h is HydroType = float
+// (sph/types.hpp:39-46), so its update runs in float arithmetic with powf
+float orc_update_h_f32(unsigned ng0, unsigned nc, float h)
+{
+    const float c0 = 1023.0f, ex = 1.0 / 10.0;
+    return h * 0.5f * std::pow(1.0f + c0 * ng0 / float(nc), ex);
+}
+
+// updateSmoothingLengthCpu<float> (sph/include/sph/update_h.hpp:13-32)
+int orc_update_smoothing_length_f32(uint32_t first, uint32_t last, unsigned ng0, const unsigned* nc, float* h,
+                                    uint64_t* keys)
+{
+    int removed = 0;
+    for (uint32_t i = first; i < last; ++i)
+    {
+        if (nc[i] <= 1)
+        {
+            if (keys) keys[i] = kRemoveKey;
+            removed = 1;
+        }
+        h[i] = orc_update_h_f32(ng0, nc[i], h[i]);
     }
     return removed;
 }

@@ -111,6 +111,7 @@ SIGNATURES = {
     "sphx_update_smoothing_length": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), _V,
                                                C.POINTER(C.c_int)]),
     "sphx_kernel_derivative_tables": (C.c_int, [_D, _V, _V]),
+    "sphx_smoothing_length_factors": (C.c_int, [_U32, _U32, _V, _V]),
     "sphx_compute_xmass": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),
                                      C.POINTER(Box)]),
     "sphx_compute_ve_def_gradh": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(VeFields),

@@ -54,13 +54,18 @@ class _Pipeline:
         groups and lists.  Returns self.d."""
         ctx, box = self.ctx, self.box
         n = x.numel()
+        for name, t in (("y", y), ("z", z)) + tuple(fields.items()):
+            if t.numel() != n:
+                raise ValueError(f"{name}: {t.numel()} elements, x has {n}")
+        # FP64 contiguous copies of the positions, used for the keys and the gather alike
+        x, y, z = (t.detach().to(torch.float64).contiguous() for t in (x, y, z))
         d = self._data(n)
         self.keys.zero_()
         hydro.compute_sfc_keys(ctx, x, y, z, box, self.keys)
         ctx.check(ctx.lib.sphx_sequence(ctx.h, hydro._ptr(self.order), n, 0))
         hydro.sort_by_key(ctx, self.keys, self.order)
         for name, src in (("x", x), ("y", y), ("z", z)):
-            hydro.gather(ctx, self.order, src.detach().contiguous(), getattr(d, name))
+            hydro.gather(ctx, self.order, src, getattr(d, name))
         for name, src in fields.items():
             hydro.gather(ctx, self.order, src.detach().to(self.dtype).contiguous(), getattr(d, name))
         leaves, counts, layout = hydro.compute_octree(ctx, self.keys, self.bucket)

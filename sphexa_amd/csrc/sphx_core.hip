@@ -46,6 +46,28 @@ void* scratch(sphx_ctx* ctx, Slot s, size_t bytes)
     return b.p;
 }
 
+int hFactors(sphx_ctx* ctx, uint32_t ng0, HFactors& out)
+{
+    const uint32_t cap = kHFactorCap;
+    char*          buf = static_cast<char*>(scratch(ctx, S_HFAC, size_t(cap) * (sizeof(double) + sizeof(float))));
+    if (!buf) return SPHX_ERR_NOMEM;
+    if (ctx->hfacNg0 != (long long)ng0)
+    {
+        std::vector<double> f64(cap);
+        std::vector<float>  f32(cap);
+        sphx_smoothing_length_factors(ng0, cap, f64.data(), f32.data());
+        SPHX_HIP(ctx, hipMemcpyAsync(buf, f64.data(), cap * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        SPHX_HIP(ctx, hipMemcpyAsync(buf + cap * sizeof(double), f32.data(), cap * sizeof(float),
+                                     hipMemcpyHostToDevice, ctx->stream));
+        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream)); // the host vectors end here
+        ctx->hfacNg0 = ng0;
+    }
+    out.f64 = reinterpret_cast<const double*>(buf);
+    out.f32 = reinterpret_cast<const float*>(buf + cap * sizeof(double));
+    out.cap = cap;
+    return SPHX_OK;
+}
+
 hipEvent_t timingEvent(sphx_ctx* ctx)
 {
     if (ctx->eventsUsed == ctx->eventPool.size())
@@ -488,6 +510,22 @@ using namespace sphx;
 // C ABI
 extern "C"
 {
+
+int sphx_smoothing_length_factors(uint32_t ng0, uint32_t count, double* f64, float* f32)
+{
+#pragma clang fp contract(off)
+    // updateH<T> (sph/kernels.hpp:27-33): T(1) + c0 * ng0 / T(nc) with c0 = 1023 and
+    // exponent T(1.0 / 10.0), evaluated in T; std::pow(float, float) is powf
+    const double c0 = 1023.0, e = 1.0 / 10.0;
+    const float  c0f = 1023.0f, ef = float(1.0 / 10.0);
+    for (uint32_t nc = 0; nc < count; ++nc)
+    {
+        if (f64) f64[nc] = std::pow(1.0 + c0 * double(ng0) / double(nc), e);
+        if (f32) f32[nc] = std::pow(1.0f + c0f * float(ng0) / float(nc), ef);
+    }
+    return SPHX_OK;
+}
+
 
 const char* sphx_version(void) { return "sphx 0.1 gfx950"; }
 

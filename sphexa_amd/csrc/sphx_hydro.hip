@@ -255,14 +255,21 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
 int sphx_kernel_derivative_tables(double sincIndex, double* whd64, float* whd32)
 {
 #pragma clang fp contract(off)
-    // powSincDerivative (sph_kernel_tables.hpp:104-108) with wharmonic_derivative_std (kernels.hpp:45-58)
+    // powSincDerivative (sph_kernel_tables.hpp:104-108) with wharmonic_derivative_std (kernels.hpp:45-58).
+    // The libm calls are those of the reference CPU path as g++ -O2 builds it: the sin and cos
+    // of wharmonic_derivative_std come from one sincos call (g++ merges them), the sin of
+    // wharmonic_std from a separate sin call.  glibc's sincos and sin round differently for a
+    // few arguments, so this choice decides a handful of the table's last bits.
     auto dW = [sincIndex](double v) {
         double s = 1.0, d = 0.0;
         if (v != 0.0)
         {
             const double Pv = M_PI_2 * v;
-            s               = std::sin(Pv) / Pv;
-            d               = s * M_PI_2 * ((std::cos(Pv) / std::sin(Pv)) - 1.0 / Pv);
+            double       sinPv, cosPv;
+            ::sincos(Pv, &sinPv, &cosPv);
+            const double sincv = sinPv / Pv;
+            d                  = sincv * M_PI_2 * ((cosPv / sinPv) - 1.0 / Pv);
+            s                  = std::sin(Pv) / Pv;
         }
         return sincIndex * std::pow(s, sincIndex - 1) * d;
     };

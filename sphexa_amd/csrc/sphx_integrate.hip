@@ -151,18 +151,14 @@ __global__ void fillU64(unsigned long long* p, unsigned long long v, uint32_t n)
     if (i < n) p[i] = v;
 }
 
-// updateH (sph/include/sph/kernels.hpp:27-33) in double
-__device__ inline double updateHd(unsigned ng0, unsigned nc, double h)
-{
-    return h * 0.5 * pow(1.0 + 1023.0 * double(ng0) / double(nc), 0.1);
-}
-
 // updateSmoothingLengthGpu semantics (sph/include/sph/update_h.hpp:13-32): new h from
-// the neighbor count, removeKey for particles with at most themselves as neighbor
+// the neighbor count (updateH, sph/kernels.hpp:27-33, in the precision of h with the
+// tabulated factor), removeKey for particles with at most themselves as neighbor.
+// status bit 0: a key was removed; bit 1: a count beyond the factor table (h unchanged)
 template<class T>
-__global__ __launch_bounds__(256) void updateHKernel(uint32_t first, uint32_t last, unsigned ng0,
+__global__ __launch_bounds__(256) void updateHKernel(uint32_t first, uint32_t last, HFactors hfac,
                                                      const uint32_t* __restrict__ nc, T* __restrict__ h,
-                                                     uint64_t* __restrict__ keys, uint32_t* removed)
+                                                     uint64_t* __restrict__ keys, uint32_t* status)
 {
     uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= last) return;
@@ -170,9 +166,11 @@ __global__ __launch_bounds__(256) void updateHKernel(uint32_t first, uint32_t la
     if (c <= 1)
     {
         if (keys) keys[i] = kRemoveKey;
-        atomicOr(removed, 1u);
+        atomicOr(status, 1u);
     }
-    h[i] = T(updateHd(ng0, c, double(h[i])));
+    T hv = h[i];
+    if (updateHTab(hfac, c, hv)) h[i] = hv;
+    else atomicOr(status, 2u);
 }
 
 } // namespace sphx
@@ -285,24 +283,29 @@ int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     if (!flag) return SPHX_ERR_NOMEM;
     fillU64<<<1, 64, 0, ctx->stream>>>(flag, 0ull, 1);
     SPHX_LAUNCH_CHECK(ctx, "fillU64");
-    uint32_t* removed = reinterpret_cast<uint32_t*>(flag);
-    unsigned  blocks  = gridFor(G.last - G.first, 256);
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED)
+        return argError(ctx, "update_smoothing_length: unknown precision");
+    HFactors hfac;
+    SPHX_TRY(hFactors(ctx, d->ng0, hfac));
+    uint32_t* status = reinterpret_cast<uint32_t*>(flag);
+    unsigned  blocks = gridFor(G.last - G.first, 256);
     {
         KTimer kt_(ctx, "updateHKernel");
         if (d->precision == SPHX_FP64)
-            updateHKernel<double><<<blocks, 256, 0, ctx->stream>>>(G.first, G.last, d->ng0, d->nc,
-                                                                    static_cast<double*>(d->h), keys, removed);
-        else if (d->precision == SPHX_MIXED)
-            updateHKernel<float><<<blocks, 256, 0, ctx->stream>>>(G.first, G.last, d->ng0, d->nc,
-                                                                   static_cast<float*>(d->h), keys, removed);
+            updateHKernel<double><<<blocks, 256, 0, ctx->stream>>>(G.first, G.last, hfac, d->nc,
+                                                                    static_cast<double*>(d->h), keys, status);
         else
-            return argError(ctx, "update_smoothing_length: unknown precision");
+            updateHKernel<float><<<blocks, 256, 0, ctx->stream>>>(G.first, G.last, hfac, d->nc,
+                                                                   static_cast<float*>(d->h), keys, status);
     }
     SPHX_LAUNCH_CHECK(ctx, "updateHKernel");
     uint32_t r = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&r, removed, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipMemcpyAsync(&r, status, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
     SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (keysRemoved) *keysRemoved = r ? 1 : 0;
+    if (keysRemoved) *keysRemoved = (r & 1u) ? 1 : 0;
+    if (r & 2u)
+        return fail(ctx, SPHX_ERR_ARG, "update_smoothing_length: a neighbor count exceeds the factor table (" +
+                                           std::to_string(kHFactorCap) + "); h left unchanged there");
     return SPHX_OK;
 }
 

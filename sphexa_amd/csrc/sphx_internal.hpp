@@ -46,6 +46,7 @@ enum Slot : int
     S_TABLE32,       // sinc table, FP32
     S_TABLE_D,       // kernel-derivative table (VE gradh loop), FP64
     S_HY_REC,        // packed neighbor-side records of the SPH loops
+    S_HFAC,          // smoothing-length update factors per neighbor count (FP64, then FP32)
     S_NUM_SLOTS
 };
 
@@ -86,6 +87,36 @@ __host__ __device__ inline void groupRange(const GroupsDev& G, uint32_t g, uint3
         s = G.first + 64u * g;
         e = (s + 64u < G.last) ? s + 64u : G.last;
     }
+}
+
+// Smoothing-length update factors of updateH (sph/kernels.hpp:27-33):
+// h' = (h * 0.5) * f(nc), f(nc) = pow(1 + 1023 ng0 / nc, 0.1).  The factor depends only on
+// the integers ng0 and nc, so it is tabulated on the host with the host libm (the pow the
+// reference CPU path calls) for nc in [0, cap): f64 in double (updateH<double>), f32 in
+// float arithmetic with powf (updateH<float>, the reference app's FP32 h).  A count at or
+// beyond cap raises an error instead of falling back to the device pow.
+constexpr uint32_t kHFactorCap = 1u << 17;
+struct HFactors
+{
+    const double* f64 = nullptr;
+    const float*  f32 = nullptr;
+    uint32_t      cap = 0;
+};
+
+template<class T>
+__device__ inline const T* hFactorsOf(const HFactors& F)
+{
+    if constexpr (sizeof(T) == 8) return F.f64;
+    else return F.f32;
+}
+
+// updateH with the tabulated factor; false (h unchanged) when nc >= F.cap
+template<class T>
+__device__ inline bool updateHTab(const HFactors& F, uint32_t nc, T& h)
+{
+    if (nc >= F.cap) return false;
+    h = (h * T(0.5)) * hFactorsOf<T>(F)[nc];
+    return true;
 }
 
 // neighbor-list layout: GPU [group][k][lane] or the reference CPU layout [i - first][k]
@@ -158,6 +189,8 @@ struct sphx_ctx
     int32_t csNumLeaves = 0;
     // computed groups
     uint32_t grpNumGroups = 0;
+    // smoothing-length factor tables in S_HFAC, built for this ng0 (-1: none yet)
+    long long hfacNg0 = -1;
 };
 
 namespace sphx
@@ -166,6 +199,9 @@ namespace sphx
 int fail(sphx_ctx* ctx, int code, const std::string& msg);
 int failHip(sphx_ctx* ctx, hipError_t e, const char* what);
 int argError(sphx_ctx* ctx, const char* what);
+
+// the smoothing-length factor tables for ng0 (built on the host on first use per ng0)
+int hFactors(sphx_ctx* ctx, uint32_t ng0, HFactors& out);
 
 // grow-only scratch; returns nullptr (and sets the error) on allocation failure
 void* scratch(sphx_ctx* ctx, Slot s, size_t bytes);

@@ -41,6 +41,7 @@ struct NsFlags
     uint32_t           overflow;   // traversal capacity exceeded
     uint32_t           badGroup;   // a group with more than 64 particles
     uint32_t           retryCount; // groups that exceeded the small traversal capacities
+    uint32_t           hfacMiss;   // a count to update h from lies beyond the factor table
     unsigned long long tests;      // exact (target, candidate) distance tests of this pass
 };
 
@@ -271,12 +272,6 @@ __global__ __launch_bounds__(64) void nsGroupRangesRetry(GroupsDev G, const doub
     }
 }
 
-// updateH (sph/kernels.hpp:27-33)
-__device__ inline double updateH(unsigned ng0, unsigned nc, double h)
-{
-    return h * 0.5 * pow(1.0 + 1023.0 * double(ng0) / double(nc), 0.1);
-}
-
 // Candidate test.  Sph = true: findNeighborsSph semantics (nc counts self, h updates,
 // redo flags) and the SPH list format below; Sph = false: findNeighbors semantics
 // (count without self, u32 lists in the reference CPU layout).
@@ -308,7 +303,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
                                                    const double* __restrict__ z, Th* __restrict__ h,
                                                    uint8_t* __restrict__ iter, const uint2* __restrict__ ranges,
                                                    const uint32_t* __restrict__ rangeCount, DBox box, uint32_t ngmax,
-                                                   uint32_t ng0, uint32_t* __restrict__ nbrs, SphLists out,
+                                                   uint32_t ng0, HFactors hfac, uint32_t* __restrict__ nbrs, SphLists out,
                                                    uint32_t* __restrict__ nc, uint32_t* __restrict__ redoList,
                                                    NsFlags* __restrict__ flags,
                                                    unsigned long long* __restrict__ groupTests)
@@ -607,9 +602,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void ns
             nc[i] = ncSph; // dataset counts are indexed by particle (sph/find_neighbors.hpp:51)
             if (outR && iter[i] < 10)
             {
-                h[i] = Th(updateH(ng0, ncSph, hi));
-                iter[i] += 1;
-                redo = true;
+                // updateH (sph/kernels.hpp:27-33) in the precision of h, factor from the table
+                Th hn = Th(hi);
+                if (updateHTab(hfac, ncSph, hn))
+                {
+                    h[i] = hn;
+                    iter[i] += 1;
+                    redo = true;
+                }
+                else
+                    flags->hfacMiss = 1;
             }
         }
         uint64_t anyRedo = ballot(redo);
@@ -643,6 +645,7 @@ __global__ void nsResetFlags(NsFlags* f, int clearErrors)
         {
             f->overflow = 0;
             f->badGroup = 0;
+            f->hfacMiss = 0;
         }
     }
 }
@@ -783,7 +786,9 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
     auto*     gtests = scratchT<unsigned long long>(ctx, S_NS_TESTS, numG);
     NsFlags*  flags  = scratchT<NsFlags>(ctx, S_SMALL, 1);
     if (!ranges || !rcnt || !iter || !redo || !retry || !gtests || !flags) return SPHX_ERR_NOMEM;
-    DBox db = makeDBox(box);
+    DBox     db = makeDBox(box);
+    HFactors hfac;
+    if (Sph) SPHX_TRY(hFactors(ctx, ng0, hfac));
 
     if (G.last > G.first)
     {
@@ -818,8 +823,8 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         {
             KTimer kt_(ctx, "nsCandidates");
             nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, iter, ranges, rcnt, db,
-                                                                    ngmax, ng0, nbrs, lists, nc, outList, flags,
-                                                                    gtests);
+                                                                    ngmax, ng0, hfac, nbrs, lists, nc, outList,
+                                                                    flags, gtests);
         }
         SPHX_LAUNCH_CHECK(ctx, "nsCandidates");
         nsSumTests<<<std::min<uint32_t>(256u, (numLaunch + 1023u) / 1024u), 256, 0, ctx->stream>>>(gtests, numLaunch,
@@ -831,6 +836,10 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         ctx->nsTests += double(hf.tests);
         if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
         if (hf.badGroup) return argError(ctx, "neighbor search: a target group holds more than 64 particles");
+        if (hf.hfacMiss)
+            return fail(ctx, SPHX_ERR_STACK,
+                        "neighbor search: a neighbor count to update h from exceeds the factor table (" +
+                            std::to_string(kHFactorCap) + ")");
         if (!Sph) break;
         numLaunch = hf.redoCount;
         list      = outList;

@@ -66,6 +66,33 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _dev(t, dtypes, n, name, optional=False, on_device=True):
+    """Checked device pointer of a tensor the library reads or writes n elements of: the
+    dtype must be one of ``dtypes`` (the kernels reinterpret the memory), the tensor
+    contiguous (the kernels index it densely), at least n long, and on a GPU (unless
+    ``on_device`` is False).  None is allowed only when ``optional``.  Raises
+    TypeError/ValueError before any device call."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: missing")
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    dtypes = dtypes if isinstance(dtypes, tuple) else (dtypes,)
+    if t.dtype not in dtypes:
+        raise TypeError(f"{name}: dtype {t.dtype}, expected {' or '.join(str(d) for d in dtypes)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: not contiguous (pass .contiguous())")
+    if t.numel() < n:
+        raise ValueError(f"{name}: {t.numel()} elements, at least {n} needed")
+    if on_device and not t.is_cuda:
+        raise ValueError(f"{name}: on {t.device}, expected a GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+_F64, _F32, _I32, _I64 = torch.float64, torch.float32, torch.int32, torch.int64
+
+
 def make_box(lo, hi, bc=BC_OPEN):
     """cstone::Box<double>: lo/hi scalars or 3-sequences, bc scalar or 3-sequence."""
     lo3 = [lo] * 3 if np.isscalar(lo) else list(lo)
@@ -164,35 +191,55 @@ def make_global_box(ctx: Context, x, y, z, previous: Box) -> Box:
     """cstone::makeGlobalBox on one rank (sfc/box_mpi.hpp:68-107): open dimensions take
     the coordinate extrema, periodic and fixed ones keep the limits of `previous`."""
     out = Box()
-    ctx.check(ctx.lib.sphx_make_global_box(ctx.h, _ptr(x), _ptr(y), _ptr(z), x.numel(), C.byref(previous),
-                                           C.byref(out)))
+    n = x.numel()
+    ctx.check(ctx.lib.sphx_make_global_box(ctx.h, _dev(x, _F64, n, "x"), _dev(y, _F64, n, "y"), _dev(z, _F64, n, "z"),
+                                           n, C.byref(previous), C.byref(out)))
     return out
+
+
+def limit_box_shrinking(fitting: Box, previous: Box, shrink_limit: float = 0.05) -> Box:
+    """cstone::limitBoxShrinking (domain/include/cstone/sfc/box.hpp:397-414): each side of the
+    fitted box moves inward by at most shrink_limit of the previous box's length (Box's own
+    xmax - xmin); the boundary types are the previous box's.  Host arithmetic in double."""
+    lx, ly, lz = previous.xmax - previous.xmin, previous.ymax - previous.ymin, previous.zmax - previous.zmin
+    return Box(min(fitting.xmin, previous.xmin + shrink_limit * lx), max(fitting.xmax, previous.xmax - shrink_limit * lx),
+               min(fitting.ymin, previous.ymin + shrink_limit * ly), max(fitting.ymax, previous.ymax - shrink_limit * ly),
+               min(fitting.zmin, previous.zmin + shrink_limit * lz), max(fitting.zmax, previous.zmax - shrink_limit * lz),
+               previous.bx, previous.by, previous.bz, 0)
 
 
 def compute_sfc_keys(ctx: Context, x, y, z, box: Box, keys=None):
     n = x.numel()
     if keys is None:
         keys = torch.zeros(n, dtype=torch.int64, device=x.device)
-    ctx.check(ctx.lib.sphx_compute_sfc_keys(ctx.h, _ptr(x), _ptr(y), _ptr(z), _ptr(keys), n, C.byref(box)))
+    ctx.check(ctx.lib.sphx_compute_sfc_keys(ctx.h, _dev(x, _F64, n, "x"), _dev(y, _F64, n, "y"),
+                                            _dev(z, _F64, n, "z"), _dev(keys, _I64, n, "keys"), n, C.byref(box)))
     return keys
 
 
 def sequence(ctx: Context, n: int, first: int = 0, device=None):
     v = torch.empty(n, dtype=torch.int32, device=device or ctx.device)
-    ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(v), n, first))
+    ctx.check(ctx.lib.sphx_sequence(ctx.h, _dev(v, _I32, n, "sequence"), n, first))
     return v
 
 
 def sort_by_key(ctx: Context, keys, values):
     """Stable in-place sort of (int64-stored uint64 keys, int32-stored uint32 values)."""
-    ctx.check(ctx.lib.sphx_sort_by_key(ctx.h, _ptr(keys), _ptr(values), keys.numel()))
+    n = keys.numel()
+    ctx.check(ctx.lib.sphx_sort_by_key(ctx.h, _dev(keys, _I64, n, "keys"), _dev(values, _I32, n, "values", True),
+                                       n))
 
 
 def gather(ctx: Context, order, src, dst=None):
+    """dst[i] = src[order[i]] for i < src.numel(); dst has src's dtype."""
+    fns = {torch.float64: ctx.lib.sphx_gather_f64, torch.float32: ctx.lib.sphx_gather_f32,
+           torch.int32: ctx.lib.sphx_gather_u32}
+    if src.dtype not in fns:
+        raise TypeError(f"gather: src dtype {src.dtype}, expected float64, float32 or int32")
     dst = torch.empty_like(src) if dst is None else dst
-    fn = {torch.float64: ctx.lib.sphx_gather_f64, torch.float32: ctx.lib.sphx_gather_f32,
-          torch.int32: ctx.lib.sphx_gather_u32}[src.dtype]
-    ctx.check(fn(ctx.h, _ptr(order), _ptr(src), _ptr(dst), src.numel()))
+    n = src.numel()
+    ctx.check(fns[src.dtype](ctx.h, _dev(order, _I32, n, "order"), _dev(src, src.dtype, n, "src"),
+                             _dev(dst, src.dtype, n, "dst"), n))
     return dst
 
 
@@ -201,7 +248,8 @@ def compute_octree(ctx: Context, keys, bucket_size: int = 64, max_count: int = 0
     """(leaves int64[L+1], counts int32[L], layout int32[L+1]) as new device tensors."""
     L = C.c_int32()
     pl, pc, py = C.c_void_p(), C.c_void_p(), C.c_void_p()
-    ctx.check(ctx.lib.sphx_compute_octree(ctx.h, _ptr(keys), keys.numel(), bucket_size, max_count, C.byref(L),
+    ctx.check(ctx.lib.sphx_compute_octree(ctx.h, _dev(keys, _I64, keys.numel(), "keys"), keys.numel(), bucket_size,
+                                          max_count, C.byref(L),
                                           C.byref(pl), C.byref(pc), C.byref(py)))
     n = L.value
     leaves = torch.empty(n + 1, dtype=torch.int64, device=keys.device)
@@ -281,8 +329,10 @@ def find_neighbors(ctx: Context, x, y, z, h, box: Box, octree: OctreeData, ngmax
     nc = torch.zeros(n, dtype=torch.int32, device=x.device)
     nb = torch.zeros(n * ngmax, dtype=torch.int32, device=x.device) if lists else None
     v = octree.view()
-    ctx.check(ctx.lib.sphx_find_neighbors(ctx.h, _ptr(x), _ptr(y), _ptr(z), _ptr(h), first, last, C.byref(box),
-                                          C.byref(v), ngmax, _ptr(nb), _ptr(nc)))
+    m = x.numel()
+    ctx.check(ctx.lib.sphx_find_neighbors(ctx.h, _dev(x, _F64, m, "x"), _dev(y, _F64, m, "y"), _dev(z, _F64, m, "z"),
+                                          _dev(h, _F64, m, "h"), first, last, C.byref(box), C.byref(v), ngmax,
+                                          _ptr(nb), _ptr(nc)))
     return (nb.view(n, ngmax) if nb is not None else None), nc
 
 
@@ -356,12 +406,19 @@ class ParticlesData:
         return self
 
     def struct(self) -> Particles:
+        """The sphx_particles view; every present field is checked (dtype, contiguity, length,
+        device) because the kernels reinterpret the raw pointers."""
         s = Particles()
         s.n, s.precision, s.ng0, s.ngmax, s.Kcour, s.K, s.gamma = (self.n, self.precision, self.ng0, self.ngmax,
                                                                    self.Kcour, self.K, self.gamma)
-        for f in ("x", "y", "z", "nc", "wh") + self.HYDRO + self.INTEGRATION:
-            t = getattr(self, f)
-            setattr(s, f, t.data_ptr() if t is not None else None)
+        fields = ("x", "y", "z", "nc", "wh") + self.HYDRO + self.INTEGRATION
+        spec = {f: (_F64 if f in ("x", "y", "z") else (_I32 if f == "nc" else self.dtype),
+                    _lib.TABLE_SIZE if f == "wh" else self.n) for f in fields}
+        for f in fields:  # layout of every field first, then the device
+            _dev(getattr(self, f), *spec[f], f"ParticlesData.{f}", optional=True, on_device=False)
+        for f in fields:
+            p = _dev(getattr(self, f), *spec[f], f"ParticlesData.{f}", optional=True)
+            setattr(s, f, p.value if p is not None else None)
         s.minDtCourant = self.minDtCourant
         return s
 
@@ -390,8 +447,10 @@ def compute_groups(ctx: Context, x, y, z, h, octree: OctreeData, box: Box, tol_f
     prec = MIXED if (h is not None and h.dtype == torch.float32) else FP64
     v = octree.view()
     g = Groups()
-    ctx.check(ctx.lib.sphx_compute_groups(ctx.h, first, last, _ptr(x), _ptr(y), _ptr(z), _ptr(h), prec, C.byref(v),
-                                          C.byref(box), tol_factor, C.byref(g)))
+    m = x.numel()
+    ctx.check(ctx.lib.sphx_compute_groups(ctx.h, first, last, _dev(x, _F64, m, "x"), _dev(y, _F64, m, "y"),
+                                          _dev(z, _F64, m, "z"), _dev(h, (_F64, _F32), m, "h", True), prec,
+                                          C.byref(v), C.byref(box), tol_factor, C.byref(g)))
     bounds = torch.empty(g.numGroups + 1, dtype=torch.int32, device=x.device)
     if g.numGroups > 0:
         ctx.d2d(bounds, g.groupStart, 4 * (g.numGroups + 1))
@@ -489,7 +548,7 @@ def update_smoothing_length(ctx: Context, d: ParticlesData, keys=None, groups=No
     s = d.struct()
     r = C.c_int()
     ctx.check(ctx.lib.sphx_update_smoothing_length(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
-                                                   _ptr(keys), C.byref(r)))
+                                                   _dev(keys, _I64, d.n, "keys", True), C.byref(r)))
     return bool(r.value)
 
 
@@ -554,8 +613,9 @@ class VeData:
     def struct(self) -> VeFields:
         v = VeFields()
         for f in self.FIELDS + self.GRAD_V + ("whd",):
-            t = getattr(self, f)
-            setattr(v, f, t.data_ptr() if t is not None else None)
+            p = _dev(getattr(self, f), self.dtype, _lib.TABLE_SIZE if f == "whd" else self.n, f"VeData.{f}",
+                     optional=True)
+            setattr(v, f, p.value if p is not None else None)
         v.alphamin, v.alphamax, v.decay = self.alphamin, self.alphamax, self.decay
         v.Atmin, v.Atmax, v.ramp = self.Atmin, self.Atmax, self.ramp
         v.avClean = int(bool(self.av_clean))
@@ -617,7 +677,8 @@ def density_adjoint(ctx: Context, d: ParticlesData, box: Box, g_rho, groups=None
     gh = torch.empty(d.n, dtype=d.dtype, device=dev) if want_h else None
     s = d.struct()
     ctx.check(ctx.lib.sphx_density_adjoint(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box),
-                                           _ptr(g_rho), _ptr(gx), _ptr(gy), _ptr(gz), _ptr(gh)))
+                                           _dev(g_rho, d.dtype, d.n, "g_rho"), _ptr(gx), _ptr(gy), _ptr(gz),
+                                           _ptr(gh)))
     return (gx, gy, gz, gh) if want_h else (gx, gy, gz)
 
 
@@ -631,7 +692,9 @@ def momentum_energy_adjoint(ctx: Context, d: ParticlesData, box: Box, g_ax=None,
     gx, gy, gz = (torch.empty(d.n, dtype=torch.float64, device=dev) for _ in range(3))
     s = d.struct()
     ctx.check(ctx.lib.sphx_momentum_energy_adjoint(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
-                                                   C.byref(box), _ptr(g_ax), _ptr(g_ay), _ptr(g_az), _ptr(g_du),
+                                                   C.byref(box), *(_dev(g, d.dtype, d.n, nm, True) for g, nm in
+                                                                   ((g_ax, "g_ax"), (g_ay, "g_ay"), (g_az, "g_az"),
+                                                                    (g_du, "g_du"))),
                                                    _ptr(gx), _ptr(gy), _ptr(gz)))
     return gx, gy, gz
 
@@ -663,15 +726,26 @@ class HydroStep:
         self.order = torch.empty(n, dtype=torch.int32, device=ctx.device)
         self.octree = None
         self.events = None
+        self.box_calls = 0
+
+    def sync_box(self, inp: dict) -> Box:
+        """The box of this call, as Domain::sync on one rank (domain/assignment.hpp:102-106):
+        periodic and fixed dimensions keep their limits; open dimensions take the particles'
+        extrema (makeGlobalBox) on the first call, limited by limitBoxShrinking against the
+        previous call's box afterwards."""
+        box = self.box
+        if not (box.bx and box.by and box.bz):
+            fitted = make_global_box(self.ctx, inp["x"], inp["y"], inp["z"], box)
+            box = self.box = fitted if self.box_calls == 0 else limit_box_shrinking(fitted, box)
+        self.box_calls += 1
+        return box
 
     def compute_forces(self, inp: dict, timed=False):
-        ctx, d, box = self.ctx, self.d, self.box
+        ctx, d = self.ctx, self.d
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(STAGES) + 1)] if timed else None
         mark = (lambda k: ev[k].record(ctx.stream)) if timed else (lambda k: None)
         mark(0)
-        if not (box.bx and box.by and box.bz):
-            # open dimensions follow the particles (Domain::sync -> makeGlobalBox)
-            box = self.box = make_global_box(ctx, inp["x"], inp["y"], inp["z"], box)
+        box = self.sync_box(inp)
         mark(1)
         self.keys.zero_()
         compute_sfc_keys(ctx, inp["x"], inp["y"], inp["z"], box, self.keys)
@@ -756,12 +830,11 @@ class VeHydroStep(HydroStep):
         self.minDtRho = math.inf
 
     def compute_forces(self, inp: dict, timed=False, dt: float = 1e-6):
-        ctx, d, box = self.ctx, self.d, self.box
+        ctx, d = self.ctx, self.d
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(VE_STAGES) + 1)] if timed else None
         mark = (lambda k: ev[k].record(ctx.stream)) if timed else (lambda k: None)
         mark(0)
-        if not (box.bx and box.by and box.bz):
-            box = self.box = make_global_box(ctx, inp["x"], inp["y"], inp["z"], box)
+        box = self.sync_box(inp)
         mark(1)
         self.keys.zero_()
         compute_sfc_keys(ctx, inp["x"], inp["y"], inp["z"], box, self.keys)

@@ -43,17 +43,43 @@ def test_kernel_tables_match_oracle():
 
 
 def test_kernel_derivative_tables_match_oracle():
-    """The library's host table of the kernel derivative: the oracle's (which equals the
-    compiled reference's bit for bit) to 1e-14 relative; the sin/cos/pow calls of the two host
-    compilers round differently in a handful of the 20000 entries."""
+    """The library's host table of the kernel derivative equals the oracle's bit for bit, and
+    the compiled reference's where oracle/_ref is built (its sin/cos come from the sincos
+    call that g++ makes of them); the FP32 table is the FP64 one rounded."""
     from sphexa_amd.hydro import kernel_derivative_tables
 
     w64, w32 = kernel_derivative_tables(6.0)
     o64 = oracle.Cpu("oracle").kernel_derivative_table(6.0)
-    nz = o64 != 0
-    assert np.max(np.abs(w64[nz] - o64[nz]) / np.abs(o64[nz])) < 1e-14
-    assert np.array_equal(w64 == 0, o64 == 0)
+    assert np.array_equal(w64, o64)
+    if oracle.available("ref"):
+        assert np.array_equal(w64, oracle.Cpu("ref").kernel_derivative_table(6.0))
     assert np.max(np.abs(w32 - o64)) < 1e-6 * np.max(np.abs(o64))
+
+
+@pytest.mark.parametrize("ng0", [100, 0, 37, 2047])
+def test_smoothing_length_factors_match_reference(ng0):
+    """The host factors every h update of the library applies, h' = (h * 0.5) * f[nc], give the
+    oracle's (and the compiled reference's) updateSmoothingLengthCpu bit for bit, in double
+    and in float, for every count of the table."""
+    lib = _lib.load()
+    cap = 1 << 17
+    f64 = np.zeros(cap, np.float64)
+    f32 = np.zeros(cap, np.float32)
+    assert lib.sphx_smoothing_length_factors(ng0, cap, f64.ctypes.data, f32.ctypes.data) == 0
+    nc = np.arange(1, cap, dtype=np.uint32)
+    rng = np.random.default_rng(ng0)
+    h = 0.01 + rng.random(len(nc)) * 0.02
+    libs = [oracle.Cpu("oracle")] + ([oracle.Cpu("ref")] if oracle.available("ref") else [])
+    for dtype, f in ((np.float64, f64), (np.float32, f32)):
+        hv = h.astype(dtype)
+        mine = (hv * dtype(0.5)) * f[1:]
+        assert mine.dtype == dtype
+        for cpu in libs:
+            want, _, _ = cpu.update_smoothing_length(ng0, nc, hv, np.zeros(len(nc), np.uint64), dtype=dtype)
+            assert np.array_equal(mine, want), (cpu.pre, dtype)
+    # nc = 0 (never produced by a search: nc counts the particle itself): the reference's
+    # division by zero, inf for ng0 > 0 and NaN for ng0 = 0, in both tables
+    assert (np.isinf(f64[0]) and np.isinf(f32[0])) if ng0 else (np.isnan(f64[0]) and np.isnan(f32[0]))
 
 
 def test_octree_sizes():

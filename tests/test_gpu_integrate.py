@@ -8,12 +8,12 @@ Tolerances:
   <= 1e-14 of the field's largest magnitude there (a 1-ulp factor difference, amplified
   where the velocity update cancels);
 * the acceleration time step: bit-exact (the minimum is exact, the root is taken on the host);
-* updateH: device pow against glibc pow, <= 4.5e-16 relative (2 ulp); removeKey marks bit-exact;
+* updateH: bit-exact in both instantiations (double, and float for MIXED h), the factor
+  pow(1 + 1023 ng0 / nc, 0.1) being tabulated on the host; removeKey marks bit-exact;
 * MIXED (FP32 state fields): FP64 arithmetic on the FP32 values, so the FP64 oracle's
   result on the same values, rounded to FP32, bit for bit (positions FP64, bit for bit);
-* three full time steps of a Sedov lattice (FP64-exact loops): x, v, u, h within 1e-12
-  relative of the oracle's three steps (h differs by device pow at 1 ulp, which can move a
-  smoothing length across a table node; the loops themselves are bit-identical).
+* three full time steps of a Sedov lattice (FP64-exact loops; std and VE propagators):
+  the state and the time steps bit for bit equal to the oracle's three steps.
 """
 import numpy as np
 import pytest
@@ -172,10 +172,11 @@ def test_update_smoothing_length(ctx, precision):
     d.nc = torch.from_numpy(nc.view(np.int32)).to("cuda")
     kd = torch.from_numpy(keys.view(np.int64)).to("cuda")
     assert not hydro.update_smoothing_length(ctx, d, kd)
-    want, _, _ = CPU.update_smoothing_length(100, nc, hd)
-    got = d.h.double().cpu().numpy()
-    tol = 4.5e-16 if precision == 0 else 6e-8
-    assert rel_err(got, want if precision == 0 else want.astype(np.float32)) <= tol
+    # bit for bit: the double instantiation of updateH, or the float one of the reference app
+    want, _, _ = CPU.update_smoothing_length(100, nc, d.h.new_tensor(hd).cpu().numpy() if precision else hd,
+                                             dtype=np.float64 if precision == 0 else np.float32)
+    got = d.h.cpu().numpy()
+    assert got.dtype == want.dtype and np.array_equal(got, want)
     assert np.array_equal(kd.cpu().numpy().view(np.uint64), keys)
     nc[[5, 99_999]] = [1, 0]
     d.nc = torch.from_numpy(nc.view(np.int32)).to("cuda")
@@ -184,11 +185,16 @@ def test_update_smoothing_length(ctx, precision):
     assert k[5] == k[99_999] == 2 ** 63 and np.array_equal(np.delete(k, [5, 99_999]), np.delete(keys, [5, 99_999]))
 
 
-def oracle_time_steps(inp, box, steps, ts):
-    """The reference's propagator on the CPU: sync (sort), computeForces, integrate."""
+def oracle_time_steps(inp, box, steps, ts, ngmax=150):
+    """The reference's propagator on the CPU: sync (box, sort), computeForces, integrate.
+    Open dimensions take the particles' extrema on the first step and are limited by
+    limitBoxShrinking afterwards (domain/assignment.hpp:102-106).  Returns (state, ts, box)."""
     st = {k: v.copy() for k, v in inp.items()}
+    prev = None
     for _ in range(steps):
-        r = oracle_step(st, box)
+        fit = CPU.make_global_box(st["x"], st["y"], st["z"], box if prev is None else prev)
+        box = prev = fit if prev is None else CPU.limit_box_shrinking(fit, prev)
+        r = oracle_step(st, box, ngmax=ngmax)
         s = {k: v.copy() for k, v in r["sorted"].items()}
         s.update(h=r["h"], ax=r["ax"], ay=r["ay"], az=r["az"], du=r["du"])
         t, _ = CPU.compute_timestep(ts, min_dt_courant=r["minDt"])
@@ -197,7 +203,7 @@ def oracle_time_steps(inp, box, steps, ts):
         s["h"], _, removed = CPU.update_smoothing_length(100, r["nc"], s["h"])
         assert not removed
         st = {k: s[k] for k in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u", "x_m1", "y_m1", "z_m1", "du_m1")}
-    return st, ts
+    return st, ts, box
 
 
 def test_sedov_time_steps_fp64_exact(ctx):
@@ -210,7 +216,7 @@ def test_sedov_time_steps_fp64_exact(ctx):
     for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
         inp[k] = np.zeros(n)
     ts0 = [1e-6, 1e-6, 0.0]
-    want, ts_want = oracle_time_steps(inp, box, 3, ts0)
+    want, ts_want, _ = oracle_time_steps(inp, box, 3, ts0)
 
     ctx.set_fp64_exact(True)
     try:
@@ -222,14 +228,86 @@ def test_sedov_time_steps_fp64_exact(ctx):
         torch.cuda.synchronize()
     finally:
         ctx.set_fp64_exact(False)
-    assert sim.ts.minDt == pytest.approx(ts_want[0], rel=1e-12)
-    assert sim.ts.ttot == pytest.approx(ts_want[2], rel=1e-12)
+    # FP64-exact loops, positions without contraction and tabulated h factors: the state and
+    # the time steps equal the oracle's bit for bit after three steps
+    assert (sim.ts.minDt, sim.ts.minDt_m1, sim.ts.ttot) == tuple(ts_want)
     got = {k: v.double().cpu().numpy() for k, v in sim.state.items()}
-    for k in ("x", "y", "z", "h", "u", "x_m1", "du_m1"):
-        assert rel_err(got[k], want[k], floor=1e-12 * np.max(np.abs(want[k]))) <= 1e-12, k
-    for k in ("vx", "vy", "vz"):
-        assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * np.max(np.abs(want[k])), k
+    for k in ("x", "y", "z", "h", "u", "x_m1", "y_m1", "z_m1", "du_m1", "vx", "vy", "vz"):
+        assert np.array_equal(got[k], want[k]), k
     assert np.max(np.abs(want["vx"])) > 0  # the blast has started to move particles
+
+
+def test_evrard_time_steps_open_box_fp64_exact(ctx):
+    """Three time steps of an Evrard-like 1/r cloud in an open box (20k particles, ngmax 512):
+    the box refitted every step (limitBoxShrinking from the second), the h iteration of the
+    search and updateSmoothingLength, FP64-exact loops; the box, the time steps and the state
+    equal the oracle's bit for bit."""
+    from sphexa_amd import hydro, workloads
+    from sphexa_amd._lib import FP64
+
+    print("case evrard 20k open box three time steps", flush=True)
+    f, (lo, hi, _) = workloads.evrard(20_000, seed=7)
+    n = len(f["x"])
+    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
+        f[k] = np.zeros(n)
+    box = oracle.Box.make(list(lo), list(hi), [oracle.BC_OPEN] * 3)
+    ts0 = [1e-4, 1e-4, 0.0]
+    want, ts_want, box_want = oracle_time_steps(f, box, 3, ts0, ngmax=512)
+
+    ctx.set_fp64_exact(True)
+    try:
+        state = {k: torch.from_numpy(v).to("cuda") for k, v in f.items()}
+        sim = hydro.Simulation(ctx, state, hbox(box), FP64, ngmax=512,
+                               ts=hydro.TimestepState(minDt=ts0[0], minDt_m1=ts0[1], ttot=ts0[2]))
+        for _ in range(3):
+            sim.step()
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_fp64_exact(False)
+    b = sim.step_obj.box
+    assert (b.xmin, b.xmax, b.ymin, b.ymax, b.zmin, b.zmax) == (box_want.xmin, box_want.xmax, box_want.ymin,
+                                                                box_want.ymax, box_want.zmin, box_want.zmax)
+    assert (sim.ts.minDt, sim.ts.minDt_m1, sim.ts.ttot) == tuple(ts_want)
+    got = {k: v.double().cpu().numpy() for k, v in sim.state.items()}
+    for k in ("x", "y", "z", "h", "u", "x_m1", "y_m1", "z_m1", "du_m1", "vx", "vy", "vz"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.max(np.abs(want["vx"])) > 0
+
+
+def test_box_shrink_limit_second_call(ctx):
+    """A second force evaluation whose particles fill a box 10 % smaller per side: the open
+    box shrinks by at most 5 % of the previous length per side (limitBoxShrinking), and the
+    keys, h, counts and density follow that box, as in the oracle."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64
+
+    print("case box shrink limit", flush=True)
+    rng = np.random.default_rng(19)
+    n = 4000
+    a = {k: rng.random(n) for k in ("x", "y", "z")}
+    h0 = 0.5 * (100 / (4 / 3 * np.pi * n)) ** (1 / 3)
+    common = dict(h=np.full(n, h0), m=np.full(n, 1.0 / n), vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n),
+                  u=np.ones(n))
+    b = {k: 0.5 + 0.8 * (v - 0.5) for k, v in a.items()}
+    box0 = oracle.Box.cube(0.0, 1.0, oracle.BC_OPEN)
+    fa = CPU.make_global_box(a["x"], a["y"], a["z"], box0)
+    fb = CPU.make_global_box(b["x"], b["y"], b["z"], fa)
+    lim = CPU.limit_box_shrinking(fb, fa)
+    assert lim.xmin < fb.xmin and lim.xmax > fb.xmax  # the limit binds
+    ref = oracle_step(dict(b, **common), lim)
+
+    step = hydro.HydroStep(ctx, n, hbox(box0), FP64)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to("cuda")
+    step.compute_forces({k: t(v) for k, v in dict(a, **common).items()})
+    d = step.compute_forces({k: t(v) for k, v in dict(b, **common).items()})
+    torch.cuda.synchronize()
+    sb = step.box
+    assert (sb.xmin, sb.xmax, sb.ymin, sb.ymax, sb.zmin, sb.zmax) == (lim.xmin, lim.xmax, lim.ymin, lim.ymax,
+                                                                      lim.zmin, lim.zmax)
+    assert np.array_equal(step.keys.cpu().numpy().view(np.uint64), ref["keys"])
+    assert np.array_equal(d.h.cpu().numpy(), ref["h"])
+    assert np.array_equal(d.nc.cpu().numpy().view(np.uint32), ref["nc"])
+    assert rel_err(d.rho.cpu().numpy(), ref["rho"]) < 1e-13
 
 
 def oracle_ve_time_steps(inp, box, steps, ts, alpha):
@@ -256,8 +334,8 @@ def oracle_ve_time_steps(inp, box, steps, ts, alpha):
 
 
 def test_ve_time_steps_fp64_exact(ctx):
-    """Three VE time steps (HydroVeProp: computeForces + integrate) against the oracle's; the
-    loops are bit-identical in FP64-exact mode, h differs by device pow (1 ulp): 1e-12."""
+    """Three VE time steps (HydroVeProp: computeForces + integrate) against the oracle's: in
+    FP64-exact mode the state and the time steps are bit-identical."""
     from sphexa_amd import hydro
     from sphexa_amd._lib import FP64
 
@@ -277,20 +355,15 @@ def test_ve_time_steps_fp64_exact(ctx):
         state["alpha"] = torch.from_numpy(alpha).to("cuda")
         sim = hydro.Simulation(ctx, state, hbox(box), FP64, propagator="ve",
                                ts=hydro.TimestepState(minDt=ts0[0], minDt_m1=ts0[1], ttot=ts0[2]))
-        # the derivative table is an input: the oracle's, as in the VE parity tests
-        sim.step_obj.ve.whd = torch.from_numpy(CPU.kernel_derivative_table()).to("cuda")
         for _ in range(3):
             sim.step()
         torch.cuda.synchronize()
     finally:
         ctx.set_fp64_exact(False)
-    assert sim.ts.minDt == pytest.approx(ts_want[0], rel=1e-12)
-    assert sim.ts.ttot == pytest.approx(ts_want[2], rel=1e-12)
+    assert (sim.ts.minDt, sim.ts.minDt_m1, sim.ts.ttot) == tuple(ts_want)
     got = {k: v.double().cpu().numpy() for k, v in sim.state.items()}
-    for k in ("x", "y", "z", "h", "u", "x_m1", "du_m1", "alpha"):
-        assert rel_err(got[k], want[k], floor=1e-12 * np.max(np.abs(want[k]))) <= 1e-12, k
-    for k in ("vx", "vy", "vz"):
-        assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * np.max(np.abs(want[k])), k
+    for k in ("x", "y", "z", "h", "u", "x_m1", "y_m1", "z_m1", "du_m1", "alpha", "vx", "vy", "vz"):
+        assert np.array_equal(got[k], want[k]), k
 
 
 def test_integration_edge_sizes(ctx):

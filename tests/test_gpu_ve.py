@@ -51,8 +51,7 @@ def cases():
     inp["vy"] = -0.2 * inp["y"] * inp["z"]
     inp["vz"] = 0.1 * np.cos(2 * np.pi * inp["z"])
     yield "sedov", inp, box, np.full(len(inp["x"]), 0.5)
-    # open box: the tight box of the particles (makeGlobalBox), fixed h (ng0 = 0: edge particles
-    # would otherwise iterate h with the device pow, 1 ulp from glibc's)
+    # open box: the tight box of the particles (makeGlobalBox); the edge particles iterate h
     inp, box = uniform_box(3000, seed=43, periodic=False, lo=-1.0, hi=1.5)
     box = CPU.make_global_box(inp["x"], inp["y"], inp["z"], box)
     yield "open", inp, box, np.random.default_rng(43).uniform(0.05, 1.0, len(inp["x"]))
@@ -62,19 +61,13 @@ def cases():
     yield "tiny", inp, box, np.random.default_rng(44).uniform(0.05, 1.0, len(inp["x"]))
 
 
-def ng0_of(box):
-    return 100 if box.bx == oracle.BC_PERIODIC else 0
 
 
 def run_gpu(ctx, inp, box, alpha, precision, dt, av_clean=False):
     from sphexa_amd import hydro
 
     n = len(inp["x"])
-    step = hydro.VeHydroStep(ctx, n, hbox(box), precision, grad_v=True, av_clean=av_clean, ng0=ng0_of(box))
-    if precision == 0:
-        # the derivative table is an input of the loops: both sides read the oracle's, whose
-        # host libm calls round like the reference's (the library's own table agrees to 1e-14)
-        step.ve.whd = torch.from_numpy(CPU.kernel_derivative_table()).to("cuda")
+    step = hydro.VeHydroStep(ctx, n, hbox(box), precision, grad_v=True, av_clean=av_clean, ng0=100)
     dtype = step.d.dtype
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda") for k, v in inp.items()}
     t = {k: (v if k in ("x", "y", "z") else v.to(dtype)) for k, v in t.items()}
@@ -117,7 +110,7 @@ def field_errors(got, want):
 def test_ve_fp64_exact(ctx, case):
     name, inp, box, alpha = next(c for c in cases() if c[0] == case)
     print("case ve fp64-exact", name, flush=True)
-    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=ng0_of(box))
+    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=100)
     ctx.set_fp64_exact(True)
     try:
         got = run_gpu(ctx, inp, box, alpha, 0, 1e-4)
@@ -135,7 +128,7 @@ def test_ve_fp64_exact(ctx, case):
 def test_ve_fp64(ctx, case):
     name, inp, box, alpha = next(c for c in cases() if c[0] == case)
     print("case ve fp64", name, flush=True)
-    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=ng0_of(box))
+    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=100)
     got = run_gpu(ctx, inp, box, alpha, 0, 1e-4)
     errs = field_errors(got, want)
     print(errs)
@@ -151,7 +144,7 @@ def test_ve_mixed(ctx, case):
     # the FP32 fields hold FP32 values on both sides
     inp = {k: (v if k in ("x", "y", "z") else v.astype(np.float32).astype(np.float64)) for k, v in inp.items()}
     alpha = alpha.astype(np.float32).astype(np.float64)
-    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=ng0_of(box))
+    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=100)
     got = run_gpu(ctx, inp, box, alpha, 1, 1e-4)
     assert np.array_equal(got["nc"], want["nc"])
     rel = {k: rel_err(got[k], want[k]) for k in ("xm", "kx", "rho", "prho", "gradh")}
@@ -179,7 +172,7 @@ def test_ve_av_cleaning_and_rho_timestep(ctx, precision):
     if precision == 1:
         inp = {k: (v if k in ("x", "y", "z") else v.astype(np.float32).astype(np.float64)) for k, v in inp.items()}
         alpha = alpha.astype(np.float32).astype(np.float64)
-    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=ng0_of(box))
+    want = oracle_ve_step(inp, box, alpha, dt=1e-4, ng0=100)
     wa = CPU.ve_momentum_energy({k: want[k] for k in want if isinstance(want[k], np.ndarray)}, want["K"],
                                 want["nbrs"], want["nc"], 150, box, av_clean=True)
     got = run_gpu(ctx, inp, box, alpha, precision, 1e-4, av_clean=True)

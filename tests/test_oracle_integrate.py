@@ -192,12 +192,28 @@ def test_positions_fixed_boundaries_equal_reference():
 
 
 @need_ref
-def test_update_h_equal_reference():
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_update_h_equal_reference(dtype):
     rng = np.random.default_rng(13)
     n = 2000
     h = 0.01 + rng.random(n) * 0.02
     nc = rng.integers(1, 400, n).astype(np.uint32)
     keys = rng.integers(0, 2 ** 62, n).astype(np.uint64)
-    a = CPU.update_smoothing_length(100, nc, h, keys)
-    b = REF.update_smoothing_length(100, nc, h, keys)
+    a = CPU.update_smoothing_length(100, nc, h, keys, dtype=dtype)
+    b = REF.update_smoothing_length(100, nc, h, keys, dtype=dtype)
+    assert a[0].dtype == dtype
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_update_h_f32_closed_form():
+    """The FP32 instantiation: h unchanged at nc = ng0 (2 * 0.5 exactly, 1024^0.1 = 2 is exact in
+    powf too), and within a few float ulp of the double closed form elsewhere."""
+    rng = np.random.default_rng(8)
+    n = 1000
+    h = (0.01 + rng.random(n) * 0.02).astype(np.float32)
+    h2, _, _ = CPU.update_smoothing_length(100, np.full(n, 100, np.uint32), h, dtype=np.float32)
+    assert np.array_equal(h2, h)
+    nc = rng.integers(2, 4000, n).astype(np.uint32)
+    h3, _, _ = CPU.update_smoothing_length(100, nc, h, dtype=np.float32)
+    want = h.astype(np.float64) * 0.5 * (1 + 1023 * 100 / nc.astype(np.float64)) ** 0.1
+    assert np.max(np.abs(h3 - want) / want) < 3e-7

@@ -147,7 +147,8 @@ int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value);
 /* statistics of the last call that produced them */
 enum
 {
-    SPHX_STAT_CANDIDATE_TESTS = 1 /* (target, candidate) pairs tested by the last neighbor search */
+    SPHX_STAT_CANDIDATE_TESTS = 1, /* (target, candidate) pairs tested by the last neighbor search */
+    SPHX_STAT_SCRATCH_BYTES   = 2  /* device bytes the context's scratch arena holds              */
 };
 int sphx_get_stat(sphx_ctx* ctx, int stat, double* value);
 
@@ -192,6 +193,15 @@ int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const
  * primitives_gpu.cu:308-339): stable ascending LSD radix sort of (key, value)
  * pairs, in place.  Scratch comes from the context. */
 int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * stable sort as sphx_sort_by_key, bit for bit, with every work buffer in one fixed device
+ * array of the library (about 24.6 MiB, one copy per GPU, held until the process ends).
+ * n must be <= 2097152 (2^21) keys, else SPHX_ERR_ARG.  It allocates nothing, never
+ * synchronises the stream and sets no memory with memset (every digit pass runs).  All
+ * calls on one GPU share the array: calls that overlap on one GPU race. */
+int sphx_sort_by_key_noalloc(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n);
 /* cstone::sequence<true> (primitives_acc.hpp:108-113): values[i] = first + i */
 int sphx_sequence(sphx_ctx* ctx, uint32_t* values, uint64_t n, uint32_t first);
 /* cstone::gather (domain/include/cstone/primitives/gather.hpp:90-98): out[i] = in[order[i]] */

@@ -23,6 +23,7 @@ OPT_TIMING = 2
 OPT_LOOP_SHAPE_FP64 = 3
 OPT_LOOP_SHAPE_MIXED = 4
 STAT_CANDIDATE_TESTS = 1
+STAT_SCRATCH_BYTES = 2
 TABLE_SIZE = 20000
 MAX_LEVEL = 21
 
@@ -81,6 +82,7 @@ SIGNATURES = {
     "sphx_kernel_tables": (C.c_int, [_D, _V, _V, _V]),
     "sphx_compute_sfc_keys": (C.c_int, [_V, _V, _V, _V, _V, _U64, C.POINTER(Box)]),
     "sphx_sort_by_key": (C.c_int, [_V, _V, _V, _U64]),
+    "sphx_sort_by_key_noalloc": (C.c_int, [_V, _V, _V, _U64]),
     "sphx_sequence": (C.c_int, [_V, _V, _U64, _U32]),
     "sphx_gather_f64": (C.c_int, [_V, _V, _V, _V, _U64]),
     "sphx_gather_f32": (C.c_int, [_V, _V, _V, _V, _U64]),

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sphx_internal.hpp"
@@ -158,9 +159,15 @@ __global__ __launch_bounds__(SCAN_THREADS) void scanAddTileOffsets(uint32_t* out
     }
 }
 
-static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, int level)
+// tiles of one scan level over n values (n + 1 outputs)
+constexpr uint64_t scanTilesOf(uint64_t n) { return (n + 1 + SCAN_TILE - 1) / SCAN_TILE; }
+
+// levelSums: caller-provided tile-sum buffers per level (numTiles + 1 entries each), or
+// nullptr for the context's scratch slots
+static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, int level,
+                     uint32_t* const* levelSums)
 {
-    uint64_t numTiles = (n + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    uint64_t numTiles = scanTilesOf(n);
     if (numTiles == 1)
     {
         scanTiles<<<1, SCAN_THREADS, 0, ctx->stream>>>(in, out, n, nullptr);
@@ -168,13 +175,12 @@ static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
         return SPHX_OK;
     }
     if (level > 2) return fail(ctx, SPHX_ERR_ARG, "scan too large");
-    Slot s = Slot(S_SCAN0 + level);
-    uint32_t* sums = scratchT<uint32_t>(ctx, s, numTiles + 1);
-    if (!sums) return SPHX_ERR_NOMEM;
+    uint32_t* sums = levelSums ? levelSums[level] : scratchT<uint32_t>(ctx, Slot(S_SCAN0 + level), numTiles + 1);
+    if (!sums) return levelSums ? fail(ctx, SPHX_ERR_ARG, "scan: no tile-sum buffer for this level") : SPHX_ERR_NOMEM;
     scanTiles<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(in, out, n, sums);
     SPHX_LAUNCH_CHECK(ctx, "scanTiles");
     // exclusive scan of the tile sums in place (numTiles entries, total discarded into sums[numTiles])
-    SPHX_TRY(scanLevel(ctx, sums, sums, numTiles, level + 1));
+    SPHX_TRY(scanLevel(ctx, sums, sums, numTiles, level + 1, levelSums));
     scanAddTileOffsets<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(out, n, sums);
     SPHX_LAUNCH_CHECK(ctx, "scanAddTileOffsets");
     return SPHX_OK;
@@ -182,7 +188,7 @@ static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
 
 int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n)
 {
-    return scanLevel(ctx, in, out, n, 0);
+    return scanLevel(ctx, in, out, n, 0, nullptr);
 }
 
 // ===========================================================================
@@ -337,26 +343,53 @@ __global__ void copyKV(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uin
     }
 }
 
-int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits)
+// Work buffers of one sort: ping-pong keys and values (n each), the per-tile digit
+// histograms (RADIX * tiles + 1) and the tile sums of their scan per level.
+struct SortWork
+{
+    uint64_t* kAlt;
+    uint32_t* vAlt;
+    uint32_t* hist;
+    uint32_t* scanSums[3];
+};
+
+// own == nullptr: buffers from the context's scratch, and digit passes that are constant
+// across all keys are detected (one read of the keys and one 8-byte copy to the host) and
+// skipped.  own != nullptr: the caller's buffers, every pass of keyBits runs, and the sort
+// is stream-ordered throughout (no host synchronisation, no memset, no timing events).
+// Both give the same result: a pass over a constant digit is the identity permutation.
+static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits,
+                         const SortWork* own)
 {
     if (n <= 1) return SPHX_OK;
     if (n >= (uint64_t(1) << 32)) return argError(ctx, "sort: n must be < 2^32");
     uint32_t numTiles = uint32_t((n + SORT_TILE - 1) / SORT_TILE);
 
-    uint64_t* kAlt = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
-    uint32_t* vAlt = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
-    uint32_t* hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
-    uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
-    if (!kAlt || (vals && !vAlt) || !hist || !vary) return SPHX_ERR_NOMEM;
+    uint64_t* kAlt;
+    uint32_t *vAlt, *hist;
+    uint64_t  varies = ~uint64_t(0);
+    if (own)
+    {
+        kAlt = own->kAlt, vAlt = vals ? own->vAlt : nullptr, hist = own->hist;
+    }
+    else
+    {
+        kAlt           = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
+        vAlt           = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
+        hist           = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
+        uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
+        if (!kAlt || (vals && !vAlt) || !hist || !vary) return SPHX_ERR_NOMEM;
 
-    // find the digit passes that change anything (one read of the keys + one 8-byte D2H)
-    SPHX_HIP(ctx, hipMemsetAsync(vary, 0, sizeof(uint64_t), ctx->stream));
-    unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8);
-    radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, vary);
-    SPHX_LAUNCH_CHECK(ctx, "radixDigitVariation");
-    uint64_t varies = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&varies, vary, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // find the digit passes that change anything (one read of the keys + one 8-byte D2H)
+        SPHX_HIP(ctx, hipMemsetAsync(vary, 0, sizeof(uint64_t), ctx->stream));
+        unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8);
+        radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, vary);
+        SPHX_LAUNCH_CHECK(ctx, "radixDigitVariation");
+        SPHX_HIP(ctx, hipMemcpyAsync(&varies, vary, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (!kAlt || (vals && !vAlt) || !hist) return argError(ctx, "sort: missing work buffer");
+    sphx_ctx* tctx = own ? nullptr : ctx; // KTimer records nothing for the caller-buffer variant
 
     uint64_t* kSrc = keys;
     uint32_t* vSrc = vals;
@@ -366,20 +399,20 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
     {
         if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
         {
-            KTimer kt_(ctx, "radixUpsweep");
+            KTimer kt_(tctx, "radixUpsweep");
             radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles);
         }
         SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
-        SPHX_TRY(exclusiveScanU32(ctx, hist, hist, uint64_t(RADIX) * numTiles));
+        SPHX_TRY(scanLevel(ctx, hist, hist, uint64_t(RADIX) * numTiles, 0, own ? own->scanSums : nullptr));
         if (vals)
             {
-                KTimer kt_(ctx, "radixScatter");
+                KTimer kt_(tctx, "radixScatter");
                 radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
                                                                           numTiles);
             }
         else
             {
-                KTimer kt_(ctx, "radixScatter");
+                KTimer kt_(tctx, "radixScatter");
                 radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
                                                                            hist, numTiles);
             }
@@ -394,6 +427,61 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
         SPHX_LAUNCH_CHECK(ctx, "copyKV");
     }
     return SPHX_OK;
+}
+
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits)
+{
+    return radixSortWith(ctx, keys, vals, n, keyBits, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free sort (sphx_sort_by_key_noalloc).  Its work buffers are one file-scope
+// device array, `scratch`, of which every GPU holds its own copy for the life of the
+// process.  The first sphx_ctx_create on a device looks up that device's address with
+// hipGetSymbolAddress (the device current) and every context on the device keeps it;
+// host code uses only that address.  Nothing in the array needs to start at zero: each
+// pass writes every histogram entry and every ping-pong element it later reads.  Calls
+// that overlap on one GPU share the array and race.
+constexpr uint64_t kNaMaxN  = uint64_t(1) << 21; // 2,097,152 keys (a 128^3 share)
+constexpr uint64_t kNaTiles = (kNaMaxN + SORT_TILE - 1) / SORT_TILE;
+constexpr uint64_t kNaScanN = uint64_t(RADIX) * kNaTiles; // histogram entries scanned
+constexpr uint64_t kNaT0 = scanTilesOf(kNaScanN), kNaT1 = scanTilesOf(kNaT0), kNaT2 = scanTilesOf(kNaT1);
+static_assert(scanTilesOf(kNaT2) == 1, "the histogram scan must fit three levels");
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kNaKeysOff  = 0;
+constexpr size_t kNaValsOff  = kNaKeysOff + align256(kNaMaxN * sizeof(uint64_t));
+constexpr size_t kNaHistOff  = kNaValsOff + align256(kNaMaxN * sizeof(uint32_t));
+constexpr size_t kNaScan0Off = kNaHistOff + align256((kNaScanN + 1) * sizeof(uint32_t));
+constexpr size_t kNaScan1Off = kNaScan0Off + align256((kNaT0 + 1) * sizeof(uint32_t));
+constexpr size_t kNaScan2Off = kNaScan1Off + align256((kNaT1 + 1) * sizeof(uint32_t));
+constexpr size_t kNaBytes    = kNaScan2Off + align256((kNaT2 + 1) * sizeof(uint32_t)); // ~24.6 MiB
+
+namespace noalloc
+{
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+} // namespace noalloc
+
+constexpr int      kNaMaxDevices = 64;
+static std::mutex  naMutex;
+static bool        naLooked[kNaMaxDevices];
+static void*       naAddr[kNaMaxDevices];
+static hipError_t  naStatus[kNaMaxDevices];
+
+// the device's address of `scratch`, looked up once per device (device current)
+static void* noallocScratchOf(int device)
+{
+    if (device < 0 || device >= kNaMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(naMutex);
+    if (!naLooked[device])
+    {
+        void*      ptr = nullptr;
+        hipError_t e   = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(noalloc::scratch));
+        if (e != hipSuccess) (void)hipGetLastError();
+        naStatus[device] = e;
+        naAddr[device]   = e == hipSuccess ? ptr : nullptr;
+        naLooked[device] = true;
+    }
+    return naAddr[device];
 }
 
 // ===========================================================================
@@ -546,6 +634,7 @@ int sphx_ctx_create(int device, void* stream, sphx_ctx** out)
     auto* ctx   = new sphx_ctx;
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(stream);
+    ctx->noallocScratch = noallocScratchOf(device);
     int cu      = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
         ctx->numCU = cu;
@@ -636,6 +725,28 @@ int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n
     if (n == 0) return SPHX_OK;
     if (!keys || !values) return argError(ctx, "sort_by_key: null pointer");
     return radixSortPairs(ctx, keys, values, n, 64);
+}
+
+int sphx_sort_by_key_noalloc(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (n > kNaMaxN)
+        return fail(ctx, SPHX_ERR_ARG, "sort_by_key_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                                           std::to_string(kNaMaxN) + " keys");
+    if (n == 0) return SPHX_OK;
+    if (!keys || !values) return argError(ctx, "sort_by_key_noalloc: null pointer");
+    if (!ctx->noallocScratch)
+        return fail(ctx, SPHX_ERR_HIP, "sort_by_key_noalloc: no device address for the work array "
+                                       "(hipGetSymbolAddress failed when the context was created)");
+    char*    base = static_cast<char*>(ctx->noallocScratch);
+    SortWork w;
+    w.kAlt        = reinterpret_cast<uint64_t*>(base + kNaKeysOff);
+    w.vAlt        = reinterpret_cast<uint32_t*>(base + kNaValsOff);
+    w.hist        = reinterpret_cast<uint32_t*>(base + kNaHistOff);
+    w.scanSums[0] = reinterpret_cast<uint32_t*>(base + kNaScan0Off);
+    w.scanSums[1] = reinterpret_cast<uint32_t*>(base + kNaScan1Off);
+    w.scanSums[2] = reinterpret_cast<uint32_t*>(base + kNaScan2Off);
+    return radixSortWith(ctx, keys, values, n, 64, &w);
 }
 
 int sphx_sequence(sphx_ctx* ctx, uint32_t* values, uint64_t n, uint32_t first)

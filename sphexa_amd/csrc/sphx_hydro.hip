@@ -119,6 +119,14 @@ int sphx_get_stat(sphx_ctx* ctx, int stat, double* value)
         *value = ctx->nsTests;
         return SPHX_OK;
     }
+    if (stat == SPHX_STAT_SCRATCH_BYTES)
+    {
+        double b = 0;
+        for (const auto& s : ctx->buf)
+            b += double(s.bytes);
+        *value = b;
+        return SPHX_OK;
+    }
     return argError(ctx, "get_stat: unknown statistic");
 }
 

@@ -191,6 +191,8 @@ struct sphx_ctx
     uint32_t grpNumGroups = 0;
     // smoothing-length factor tables in S_HFAC, built for this ng0 (-1: none yet)
     long long hfacNg0 = -1;
+    // this device's address of the allocation-free sort's work array (nullptr: unavailable)
+    void* noallocScratch = nullptr;
 };
 
 namespace sphx
@@ -247,7 +249,7 @@ struct KTimer
         : ctx(c)
         , name(n)
     {
-        if (ctx->timing && (a = timingEvent(ctx))) (void)hipEventRecord(a, ctx->stream);
+        if (ctx && ctx->timing && (a = timingEvent(ctx))) (void)hipEventRecord(a, ctx->stream);
     }
     void stop()
     {

@@ -16,7 +16,9 @@ energy, smoothing lengths) that completes a time step, the VE propagator's force
 evaluation (the reference's default propagator) on the same input, the glass relaxation of a 100^3 jittered
 lattice with the density adjoint (200 iterations), the roofline of the dominant kernel
 (live HIP-event timing; HBM traffic from the committed PMC summary in profiles/), and
-the reference CPU path timed on the host cores on the same input (one step).
+the reference CPU path timed on the host cores on the same input (one warm-up step, then
+the median of three; threads bound one per core), and each output field's error of the mixed
+step against the FP64 step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 256]
 Prints one JSON line on stdout; progress goes to stderr.
@@ -30,6 +32,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# OpenMP thread placement of the CPU baseline: one thread per core, packed.  The OpenMP
+# runtime reads these when it loads (with torch, before main runs), so they are set here.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "hydro step particles/sec (nbr search+density+IAD+mom-energy), mixed vs FP64"
 
@@ -47,6 +54,44 @@ IAD_FLOP_PER_PARTICLE = 60.0
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def valu_peak(fp64, device=0):
+    """Sustained VALU FMA rate of the device in TFLOP/s (bench_valu_probe.hip, built by
+    __graft_entry__.build() next to this file): FP64, or scalar FP32."""
+    import ctypes as C
+
+    path = os.path.join(ROOT, "bench_valu_probe.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    lib.bench_valu_peak.restype = C.c_int
+    lib.bench_valu_peak.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+    v = C.c_double()
+    st = lib.bench_valu_peak(device, int(bool(fp64)), C.byref(v))
+    if st != 0:
+        raise RuntimeError(f"bench_valu_peak failed (HIP error {st})")
+    return v.value
+
+
+# the step's output fields compared between the mixed and the FP64 run (same sorted order)
+OUTPUT_FIELDS = ("rho", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az", "du")
+
+
+def field_errors(mixed, fp64):
+    """Per output field, the mixed result against the FP64 one on the same particles:
+    max |m - f| / max |f| (the field's scale) and the RMS relative error
+    sqrt(sum (m - f)^2 / sum f^2); for rho, which is positive, also max |m - f| / f."""
+    out = {}
+    for k in OUTPUT_FIELDS:
+        m, f = mixed[k], fp64[k]
+        diff = (m - f).abs()
+        e = {"max_abs_over_scale": float(diff.max() / f.abs().max()),
+             "rms_rel": float(((m - f) ** 2).sum().sqrt() / (f ** 2).sum().sqrt())}
+        if k == "rho":
+            e["max_rel"] = float((diff / f.abs()).max())
+        out[k] = e
+    return out
 
 
 def run_precision(ctx, hydro, inp64, box, n, precision, steps, warmup, **step_kw):
@@ -249,10 +294,33 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
 
 
+def host_cpu():
+    """Model, sockets and cores of the host, and the CPUs this process may run on."""
+    model, sockets, cores = "unknown", set(), set()
+    phys = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+                sockets.add(v)
+            elif k == "core id":
+                cores.add((phys, v))
+    except OSError:
+        pass
+    return {"model": model, "sockets": len(sockets) or 1, "physical_cores": len(cores) or os.cpu_count(),
+            "logical_cpus": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(f, side, steps):
     """The reference CPU path (oracle/_ref, compiled from the reference sources) on the
-    host cores, on the same input as the GPU (host copy of f); the oracle port if the
-    reference build is absent."""
+    host cores, on the same input as the GPU (host copy of f): one untimed warm-up step,
+    then the median of `steps` timed steps, OpenMP threads bound one per core
+    (OMP_PROC_BIND=close, OMP_PLACES=cores); the oracle port if the reference build is
+    absent."""
     import ctypes as C
 
     import numpy as np
@@ -261,41 +329,49 @@ def cpu_baseline(f, side, steps):
 
     kind = "reference" if oracle.available("ref") else "port"
     n = len(f["x"])
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or cpu["cpus_allowed"])
     box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
     p = lambda a: np.ascontiguousarray(a).ctypes.data_as(C.c_void_p)
     times = []
-    stages = None
+    stages = []
     if kind == "reference":
         r = oracle.Cpu("ref")
         rho, ax, du = np.zeros(n), np.zeros(n), np.zeros(n)
         nc = np.zeros(n, np.uint32)
         st = np.zeros(11)
-        for s in range(steps):
-            t0 = time.perf_counter()
+
+        def one():
             r._hydro_step(n, p(f["x"]), p(f["y"]), p(f["z"]), p(f["h"]), p(f["m"]), p(f["vx"]), p(f["vy"]),
                           p(f["vz"]), p(f["u"]), C.byref(box), 64, 100, 150, 6.0, 5.0 / 3.0, 0.2, p(rho), p(ax),
                           p(du), p(nc), p(st))
-            times.append(time.perf_counter() - t0)
-            stages = [round(float(v), 3) for v in st]
-            log(f"cpu baseline step {s + 1}/{steps}: {times[-1]:.2f} s")
+            return [round(float(v), 3) for v in st]
     else:
         from tests.helpers import oracle_step
 
-        for s in range(steps):
-            t0 = time.perf_counter()
+        def one():
             oracle_step(f, box)
-            times.append(time.perf_counter() - t0)
-            log(f"cpu baseline (port) step {s + 1}/{steps}: {times[-1]:.2f} s")
-    t = sorted(times)[len(times) // 2]
+            return None
+    for s in range(steps + 1):
+        t0 = time.perf_counter()
+        stg = one()
+        t = time.perf_counter() - t0
+        log(f"cpu baseline ({kind}) {'warm-up' if s == 0 else f'step {s}/{steps}'}: {t:.2f} s")
+        if s:
+            times.append(t)
+            stages.append(stg)
+    k = sorted(range(steps), key=lambda q: times[q])[steps // 2]
+    t = times[k]
     out = {"value": n / t, "unit": "particles/s", "cores": threads, "kind": kind,
            "sample": f"turbulence {side}^3 = {n} particles, the GPU run's own input copied to the host (FP64), "
-                     f"median of {steps} full step(s) of the reference CPU path (keys, sort, tree, "
-                     f"findNeighborsSph, density, EOS, IAD, momentum-energy), {threads} OpenMP threads",
-           "seconds_per_step": t}
-    if stages is not None:
+                     f"one warm-up step then the median of {steps} full steps of the reference CPU path (keys, "
+                     f"sort, tree, findNeighborsSph, density, EOS, IAD, momentum-energy), {threads} OpenMP "
+                     f"threads, OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                     f"OMP_PLACES={os.environ.get('OMP_PLACES')}",
+           "seconds_per_step": t, "seconds_all_steps": [round(v, 3) for v in times], "host": cpu}
+    if stages[k] is not None:
         out["stages_s"] = dict(zip(("keys", "sort_gather", "octree", "link_centers", "neighbors", "density", "eos",
-                                    "iad", "momentum_energy"), stages))
+                                    "iad", "momentum_energy"), stages[k]))
     return out
 
 
@@ -326,7 +402,8 @@ def run_other_workload(args, ctx):
         res[prec] = dict(ms=ms, pairs=pairs, tests=tests, kernels={k: round(v[1] / args.steps, 3)
                                                                  for k, v in kernels.items()},
                          nc={"min": int(nc.min()), "median": float(np.median(nc)), "p99": float(np.percentile(nc, 99)),
-                             "max": int(nc.max())}, rho=step.d.rho.double().clone())
+                             "max": int(nc.max())},
+                         fields={k: getattr(step.d, k).double().clone() for k in OUTPUT_FIELDS})
         del step
         torch.cuda.empty_cache()
     R, F = res[MIXED], res[FP64]
@@ -338,7 +415,8 @@ def run_other_workload(args, ctx):
            "neighbors": R["nc"], "kernels_ms_per_step": R["kernels"],
            "fp64": {"value": n / (F["ms"] * 1e-3), "ms_per_step": F["ms"], "kernels_ms_per_step": F["kernels"]},
            "mixed_over_fp64": F["ms"] / R["ms"],
-           "rho_max_rel_err_mixed_vs_fp64": float(((R["rho"] - F["rho"]).abs() / F["rho"].abs()).max())}
+           "field_errors_mixed_vs_fp64": field_errors(R["fields"], F["fields"])}
+    out["rho_max_rel_err_mixed_vs_fp64"] = out["field_errors_mixed_vs_fp64"]["rho"]["max_rel"]
     print(json.dumps(out), flush=True)
 
 
@@ -349,7 +427,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--side", type=int, default=256, help="lattice side; N = side^3")
     ap.add_argument("--cpu-side", type=int, default=0, help="CPU baseline lattice side (default: --side)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU baseline steps after one warm-up")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one precision only (for profiling runs)")
@@ -396,8 +474,8 @@ def main():
     for prec in precs:
         step, ms, stages, kernels, pairs, tests = run_precision(ctx, hydro, inp64, box, n, prec, args.steps,
                                                                 args.warmup)
-        results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs,
-                             rho=step.d.rho.double().clone(), tests=tests)
+        results[prec] = dict(ms=ms, stages=stages, kernels=kernels, pairs=pairs, tests=tests,
+                             fields={k: getattr(step.d, k).double().clone() for k in OUTPUT_FIELDS})
         if not args.no_adjoint:
             results[prec]["me_adjoint_ms"] = time_me_adjoint(step)
             log(f"precision {prec}: momentum-energy adjoint {results[prec]['me_adjoint_ms']:.2f} ms")
@@ -426,6 +504,7 @@ def main():
                                f"FP32 weights and Kahan sums)",
                    "particles": n, "bucket": 64, "ng0": 100, "ngmax": 150, "parallelism": "single GPU"},
         "pairs_per_s": R["pairs"] / (R["ms"] * 1e-3),
+        "candidate_tests_per_pair": (R["tests"] or 0) / max(1, R["pairs"]),
         "stages_ms": {k: round(v, 3) for k, v in R["stages"].items()},
         "kernels_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in R["kernels"].items()},
     }
@@ -449,8 +528,12 @@ def main():
         out["fp64"]["integrate_ms"] = F["integrate"]["ms"]
         out["fp64"]["step_totals"] = step_totals(F, FP64)
         out["mixed_over_fp64"] = F["ms"] / R["ms"]
-        rf, rm = results[FP64]["rho"], results[MIXED]["rho"]
-        out["rho_max_rel_err_mixed_vs_fp64"] = float(((rm - rf).abs() / rf.abs()).max().item())
+        errs = field_errors(results[MIXED]["fields"], results[FP64]["fields"])
+        out["rho_max_rel_err_mixed_vs_fp64"] = errs["rho"]["max_rel"]
+        out["field_errors_mixed_vs_fp64"] = errs
+        out["field_errors_mixed_vs_fp64"]["note"] = (
+            "the mixed step's outputs against the FP64 step's on the same input, same sorted order; "
+            "max_abs_over_scale = max|mixed - fp64| / max|fp64|, rms_rel = ||mixed - fp64|| / ||fp64||")
     out["integrate"] = R["integrate"]
     out["time_step_ms"] = R["ms"] + R["integrate"]["ms"]
     if "me_adjoint_ms" in R:
@@ -471,7 +554,7 @@ def main():
                                    f"{ve_steps} timed steps")
     out["roofline"] = roofline(R["kernels"], R["pairs"], n, args.steps, main_prec, R["tests"])
     # the device's sustained FMA rates, measured beside the published peaks
-    out["roofline"]["valu_measured_TFLOPs"] = {"fp32_scalar": ctx.valu_peak(MIXED), "fp64": ctx.valu_peak(FP64)}
+    out["roofline"]["valu_measured_TFLOPs"] = {"fp32_scalar": valu_peak(False), "fp64": valu_peak(True)}
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"],
                                                   f"turbulence{args.side}_{'mixed' if main_prec == MIXED else 'fp64'}")
     if out["roofline"]["traffic"]:

@@ -160,11 +160,6 @@ int sphx_get_stat(sphx_ctx* ctx, int stat, double* value);
 int sphx_timing_summary(sphx_ctx* ctx, int cap, const char** names, int* launches, double* totalMs,
                         int* numEntries);
 
-/* No counterpart in the reference: the sustained VALU FMA rate of the device (TFLOP/s,
- * FMA = 2 flops) for SPHX_FP64 or SPHX_MIXED (= scalar FP32), from 8 independent FMA
- * chains per lane on every CU; checks the published vector peaks the rooflines use.
- * Synchronises the stream. */
-int sphx_valu_peak(sphx_ctx* ctx, int precision, double* tflops);
 
 /* ---- kernel tables ------------------------------------------------------- */
 /* Host-side table and normalisation: replaces ParticlesData::createTables

@@ -127,7 +127,6 @@ SIGNATURES = {
                                                   C.POINTER(VeFields), C.POINTER(Box)]),
     "sphx_rho_timestep": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles), C.POINTER(VeFields), _D,
                                     C.POINTER(_D)]),
-    "sphx_valu_peak": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_make_global_box": (C.c_int, [_V, _V, _V, _V, _U64, C.POINTER(Box), C.POINTER(Box)]),
     "sphx_get_stat": (C.c_int, [_V, C.c_int, C.POINTER(_D)]),
     "sphx_timing_summary": (C.c_int, [_V, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(_D),

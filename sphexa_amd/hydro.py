@@ -148,12 +148,6 @@ class Context:
         self.check(self.lib.sphx_get_stat(self.h, _lib.STAT_CANDIDATE_TESTS, C.byref(v)))
         return v.value
 
-    def valu_peak(self, precision: int) -> float:
-        """Sustained VALU FMA rate in TFLOP/s (FP64, or scalar FP32 for MIXED)."""
-        v = C.c_double()
-        self.check(self.lib.sphx_valu_peak(self.h, precision, C.byref(v)))
-        return v.value
-
     def synchronize(self):
         self.check(self.lib.sphx_ctx_synchronize(self.h))
 

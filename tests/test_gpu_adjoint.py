@@ -288,3 +288,117 @@ def test_density_adjoint_vs_bruteforce_autograd(ctx):
     errs = [_err(a, b.cpu().double()) for a, b in zip(got, ref)]
     print("errors", errs, flush=True)
     assert max(errs) < 1e-11
+
+
+def test_density_adjoint_coincident_particles(ctx):
+    """Two particles at the same position are neighbors (the search excludes only i itself):
+    their pair contributes W(0) to the density and nothing to the gradient, so the adjoint
+    stays finite and matches central differences of the oracle's density."""
+    import oracle
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+
+    print("case density_adjoint_coincident", flush=True)
+    inp, box = uniform_box(3000, seed=94)
+    for k in ("x", "y", "z"):
+        inp[k][5] = inp[k][17]
+    hb = hydro.make_box(box.xmin, box.xmax, box.bx)
+    op = autodiff.DensityOp(ctx, hb, FP64, ngmax=256)
+    xyz = [torch.tensor(inp[k], device=ctx.device).requires_grad_(True) for k in "xyz"]
+    rho = op(*xyz, torch.tensor(inp["h"], device=ctx.device), torch.tensor(inp["m"], device=ctx.device))
+    grads = [g.cpu().numpy() for g in torch.autograd.grad(((rho - rho.mean()) ** 2).sum(), xyz)]
+    assert all(np.all(np.isfinite(g)) for g in grads)
+    scale = max(np.abs(g).max() for g in grads)
+    d, order = op.pipe.d, op.pipe.order.long().cpu().numpy()
+    nb = hydro.export_neighbors(ctx, d).cpu().numpy().view(np.uint32)
+    nc = d.nc.cpu().numpy().view(np.uint32)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    assert inv[17] in nb[inv[5], :nc[inv[5]] - 1]  # the two are each other's neighbors
+    cpu = oracle.Cpu("oracle")
+    wh, _, K = cpu.kernel_tables()
+    s = {k: np.ascontiguousarray(inp[k][order]) for k in ("x", "y", "z", "h", "m")}
+
+    def loss(f):
+        r = cpu.density(nb, nc, d.ngmax, f["x"], f["y"], f["z"], s["h"], s["m"], wh, K, box)
+        return float(np.sum((r - r.mean()) ** 2))
+
+    worst, delta = 0.0, 2e-9
+    for p in [5, 17] + list(np.random.default_rng(95).choice(3000, 6, replace=False)):
+        for a, key in enumerate("xyz"):
+            plus, minus = dict(s), dict(s)
+            plus[key], minus[key] = s[key].copy(), s[key].copy()
+            plus[key][inv[p]] += delta
+            minus[key][inv[p]] -= delta
+            fd = (loss(plus) - loss(minus)) / (2 * delta)
+            worst = max(worst, abs(fd - grads[a][p]) / scale)
+    print("worst", worst, flush=True)
+    assert worst < 1e-6
+
+
+def test_momentum_energy_op_autograd_unsorted_inputs(ctx):
+    """MomentumEnergyOp on caller-order inputs: the forward equals the hydro step's outputs
+    mapped back through the sort order, torch.autograd.grad equals the step's momentum-energy
+    VJP mapped back (to 1e-11, atomic summation order), and both match central differences of
+    the oracle's momentum-energy loop on the op's own lists (<= 1e-6 of the gradient scale)."""
+    import oracle
+    from sphexa_amd import autodiff, hydro
+    from sphexa_amd._lib import FP64
+
+    print("case momentum_energy_op", flush=True)
+    n = 5000
+    inp, box = uniform_box(n, seed=96)
+    hb = hydro.make_box(box.xmin, box.xmax, box.bx)
+    step = hydro.HydroStep(ctx, n, hb, FP64, ngmax=256)
+    d = step.compute_forces({k: torch.tensor(v, device=ctx.device) for k, v in inp.items()})
+    torch.cuda.synchronize()
+    order = step.order.long()
+    # caller-order copies of every input of the loop (h after the step's iteration)
+    fields = {}
+    for k in ME_F:
+        t = torch.empty_like(getattr(d, k))
+        t.index_copy_(0, order, getattr(d, k))
+        fields[k] = t
+    xyz = [torch.tensor(inp[k], device=ctx.device).requires_grad_(True) for k in "xyz"]
+    op = autodiff.MomentumEnergyOp(ctx, hb, FP64, ngmax=256)
+    out = op(*xyz, **fields)
+    for k, o in zip(("ax", "ay", "az", "du"), out):
+        want = getattr(d, k)
+        got = o.detach()[order]
+        assert float((got - want).abs().max() / want.abs().max()) < 1e-12, k
+    Wt = torch.from_numpy(np.random.default_rng(97).normal(size=(4, n))).to(ctx.device)
+    loss = sum((Wt[k] * out[k]).sum() for k in range(4))
+    grads = torch.autograd.grad(loss, xyz)
+    vjp = step.momentum_energy_vjp(*[Wt[k][order] for k in range(4)])
+    for g, v in zip(grads, vjp):
+        assert _err(g[order], v.cpu().double()) < 1e-11
+    # central differences of the oracle over the op's lists and sorted fields
+    pd, porder = op.pipe.d, op.pipe.order.long().cpu().numpy()
+    nb = hydro_export(ctx, pd)
+    nc = pd.nc.cpu().numpy().view(np.uint32)
+    f = {k: getattr(pd, k).cpu().double().numpy() for k in ("x", "y", "z") + ME_F}
+    cs = [f[k] for k in ("c11", "c12", "c13", "c22", "c23", "c33")]
+    W = Wt.cpu().numpy()[:, porder]
+    cpu = oracle.Cpu("oracle")
+    owh, _, K = cpu.kernel_tables()
+
+    def oloss(g):
+        o = cpu.momentum_energy(nb, nc, pd.ngmax, g["x"], g["y"], g["z"], f["vx"], f["vy"], f["vz"], f["h"], f["m"],
+                                f["rho"], f["p"], f["c"], cs, owh, K, 0.2, box)
+        return float(sum(np.dot(W[k], o[k]) for k in range(4)))
+
+    gnp = [g.cpu().numpy() for g in grads]
+    scale = max(np.abs(g).max() for g in gnp)
+    inv = np.empty_like(porder)
+    inv[porder] = np.arange(n)
+    worst, delta = 0.0, 2e-9
+    for p in np.random.default_rng(98).choice(n, 12, replace=False):
+        for a, key in enumerate("xyz"):
+            plus, minus = dict(f), dict(f)
+            plus[key], minus[key] = f[key].copy(), f[key].copy()
+            plus[key][inv[p]] += delta
+            minus[key][inv[p]] -= delta
+            fd = (oloss(plus) - oloss(minus)) / (2 * delta)
+            worst = max(worst, abs(fd - gnp[a][p]) / scale)
+    print("worst", worst, flush=True)
+    assert worst < 1e-6

@@ -295,7 +295,9 @@ def calibrate_copy(nbytes):
 
 
 def host_cpu():
-    """Model, sockets and cores of the host, and the CPUs this process may run on."""
+    """Model, sockets, physical cores and logical CPUs of the host (/proc/cpuinfo).  The
+    process's affinity mask is not reported: with OMP_PROC_BIND set, the OpenMP runtime
+    binds the initial thread to one core when it loads."""
     model, sockets, cores = "unknown", set(), set()
     phys = None
     try:
@@ -312,7 +314,7 @@ def host_cpu():
     except OSError:
         pass
     return {"model": model, "sockets": len(sockets) or 1, "physical_cores": len(cores) or os.cpu_count(),
-            "logical_cpus": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0))}
+            "logical_cpus": os.cpu_count()}
 
 
 def cpu_baseline(f, side, steps):
@@ -330,7 +332,7 @@ def cpu_baseline(f, side, steps):
     kind = "reference" if oracle.available("ref") else "port"
     n = len(f["x"])
     cpu = host_cpu()
-    threads = int(os.environ.get("OMP_NUM_THREADS") or cpu["cpus_allowed"])
+    threads = int(os.environ.get("OMP_NUM_THREADS") or cpu["physical_cores"])
     box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
     p = lambda a: np.ascontiguousarray(a).ctypes.data_as(C.c_void_p)
     times = []

@@ -92,3 +92,16 @@ def oracle_ve_step(inp, box, alpha, dt, ng0=100, ngmax=150, bucket=64, gamma=5.0
     f["ax"], f["ay"], f["az"], f["du"], f["minDt"] = cpu.ve_momentum_energy(f, K, nb, nc, ngmax, box, Kcour=Kcour)
     f.update(nc=nc, nbrs=nb, order=order, K=K)
     return f
+
+
+def lists_digest(chunks):
+    """SHA-256 over the stored neighbors of each particle in order, nc - 1 per particle (the
+    lists are complete: ngmax exceeds every count), as u32 little-endian.  `chunks` yields
+    (rows [k, ngmax], nc [k]) for consecutive particles, as numpy arrays."""
+    import hashlib
+
+    hl = hashlib.sha256()
+    for nb, nc in chunks:
+        keep = np.arange(nb.shape[1])[None, :] < (nc.astype(np.int64) - 1)[:, None]
+        hl.update(np.ascontiguousarray(nb[keep], dtype="<u4").tobytes())
+    return np.frombuffer(hl.digest(), np.uint8)

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.helpers import oracle_step, sedov_lattice, uniform_box
+from tests.helpers import lists_digest, oracle_step, sedov_lattice, uniform_box
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 CASES = {
@@ -45,3 +45,33 @@ def test_oracle_matches_reference_golden(name):
 def test_sedov_lattice_neighbor_count():
     g = np.load(os.path.join(GOLDEN, "sedov30.npz"))
     assert (g["nc"] == 93).all()  # 92 lattice neighbors within 2h plus self, no distance ties
+
+
+def test_oracle_matches_uniform1m_golden():
+    """The uniform-random configuration at full size (1M particles, ~800 neighbors): the
+    oracle's sorted keys, counts and complete lists hash to the compiled reference's."""
+    import oracle
+    from sphexa_amd import workloads
+
+    g = np.load(os.path.join(GOLDEN, "uniform1m.npz"))
+    n, seed = int(g["n"][0]), int(g["seed"][0])
+    ng0, ngmax = int(g["ng0"][0]), int(g["ngmax"][0])
+    cpu = oracle.Cpu("oracle")
+    f, _ = workloads.uniform_random(n, seed=seed)
+    box = oracle.Box.cube(0.0, 1.0, oracle.BC_PERIODIC)
+    keys = cpu.sfc_keys(f["x"], f["y"], f["z"], box)
+    keys, order = cpu.sort_by_key(keys, np.arange(n, dtype=np.uint32))
+    assert np.array_equal(sha(keys), g["sha_keys"])
+    s = {k: np.ascontiguousarray(f[k][order]) for k in ("x", "y", "z", "h")}
+    t = cpu.tree(keys, box)
+    ncs = []
+
+    def chunks():
+        for c0 in range(0, n, 100_000):
+            nb, nc, _ = cpu.find_neighbors_sph(s["x"], s["y"], s["z"], s["h"], box, t, ng0, ngmax, c0,
+                                               min(n, c0 + 100_000))
+            ncs.append(nc)
+            yield nb, nc
+
+    assert np.array_equal(lists_digest(chunks()), g["sha_lists"])
+    assert np.array_equal(sha(np.concatenate(ncs)), g["sha_nc"])

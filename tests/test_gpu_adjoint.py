@@ -340,7 +340,14 @@ def test_momentum_energy_op_autograd_unsorted_inputs(ctx):
     """MomentumEnergyOp on caller-order inputs: the forward equals the hydro step's outputs
     mapped back through the sort order, torch.autograd.grad equals the step's momentum-energy
     VJP mapped back (to 1e-11, atomic summation order), and both match central differences of
-    the oracle's momentum-energy loop on the op's own lists (<= 1e-6 of the gradient scale)."""
+    the oracle's momentum-energy loop on the op's own lists (<= 1e-6 of the gradient scale).
+
+    The table-linear kernel has a kink at every table node (and the viscosity one at w = 0),
+    where the derivative is undefined; a central difference whose step straddles one mixes two
+    slopes.  Each sample is therefore differenced with two steps, 2e-9 and 5e-10 (outputs
+    differenced element-wise, so round-off stays far below 1e-8): where they disagree by more
+    than 1e-8 of the scale a kink lies within the step and the sample is skipped; at least 24
+    of the 36 samples must be clean."""
     import oracle
     from sphexa_amd import autodiff, hydro
     from sphexa_amd._lib import FP64
@@ -382,23 +389,33 @@ def test_momentum_energy_op_autograd_unsorted_inputs(ctx):
     cpu = oracle.Cpu("oracle")
     owh, _, K = cpu.kernel_tables()
 
-    def oloss(g):
-        o = cpu.momentum_energy(nb, nc, pd.ngmax, g["x"], g["y"], g["z"], f["vx"], f["vy"], f["vz"], f["h"], f["m"],
-                                f["rho"], f["p"], f["c"], cs, owh, K, 0.2, box)
-        return float(sum(np.dot(W[k], o[k]) for k in range(4)))
+    def outputs(g):
+        return cpu.momentum_energy(nb, nc, pd.ngmax, g["x"], g["y"], g["z"], f["vx"], f["vy"], f["vz"], f["h"],
+                                   f["m"], f["rho"], f["p"], f["c"], cs, owh, K, 0.2, box)[:4]
 
     gnp = [g.cpu().numpy() for g in grads]
     scale = max(np.abs(g).max() for g in gnp)
     inv = np.empty_like(porder)
     inv[porder] = np.arange(n)
-    worst, delta = 0.0, 2e-9
+
+    def central(key, q, delta):
+        # outputs differenced element-wise before the weighted sum: the particles the move does
+        # not reach contribute exactly zero, so no large total loss cancels
+        plus, minus = dict(f), dict(f)
+        plus[key], minus[key] = f[key].copy(), f[key].copy()
+        plus[key][q] += delta
+        minus[key][q] -= delta
+        op_, om = outputs(plus), outputs(minus)
+        return sum(float(np.dot(W[k], op_[k] - om[k])) for k in range(4)) / (2 * delta)
+
+    worst, clean = 0.0, 0
     for p in np.random.default_rng(98).choice(n, 12, replace=False):
         for a, key in enumerate("xyz"):
-            plus, minus = dict(f), dict(f)
-            plus[key], minus[key] = f[key].copy(), f[key].copy()
-            plus[key][inv[p]] += delta
-            minus[key][inv[p]] -= delta
-            fd = (oloss(plus) - oloss(minus)) / (2 * delta)
+            fd, fd_small = central(key, inv[p], 2e-9), central(key, inv[p], 5e-10)
+            if abs(fd - fd_small) > 1e-8 * scale:
+                continue  # a kink within the step
+            clean += 1
             worst = max(worst, abs(fd - gnp[a][p]) / scale)
-    print("worst", worst, flush=True)
+    print("worst", worst, "clean samples", clean, flush=True)
+    assert clean >= 24
     assert worst < 1e-6

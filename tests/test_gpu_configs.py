@@ -147,6 +147,43 @@ def test_uniform_800_neighbors_plain_search_bit_exact(ctx):
         assert np.array_equal(nb[i, :nc[i]], ref_nb[i, :nc[i]]), i
 
 
+def test_uniform_1M_lists_match_reference_fixture(ctx):
+    """Full size (1M, ~800 neighbors, ngmax 1024, FP64 step): the sorted keys, the counts and
+    every particle's complete neighbor list hash to the compiled reference's
+    (tests/golden/uniform1m.npz), and 64 sampled rows equal the fixture's."""
+    import os
+
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64
+    from tests.helpers import lists_digest
+
+    print("case uniform800 full-size lists vs reference fixture", flush=True)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "uniform1m.npz"))
+    n, seed = int(g["n"][0]), int(g["seed"][0])
+    f, spec = workloads.uniform_random(n, seed=seed)
+    obox, hbox = _boxes(spec)
+    step, d = _gpu_step(ctx, f, hbox, FP64, ngmax=int(g["ngmax"][0]), ng0=int(g["ng0"][0]))
+    sha = lambda a: np.frombuffer(__import__("hashlib").sha256(np.ascontiguousarray(a).tobytes()).digest(), np.uint8)
+    assert np.array_equal(sha(step.keys.cpu().numpy().view(np.uint64)), g["sha_keys"])
+    nc = d.nc.cpu().numpy().view(np.uint32)
+    assert np.array_equal(sha(nc), g["sha_nc"])
+    full = hydro.export_neighbors(ctx, d)  # [n, ngmax] on the device
+
+    def chunks():
+        for c0 in range(0, n, 100_000):
+            c1 = min(n, c0 + 100_000)
+            yield full[c0:c1].cpu().numpy().view(np.uint32), nc[c0:c1]
+
+    assert np.array_equal(lists_digest(chunks()), g["sha_lists"])
+    idx = g["sample_idx"]
+    rows = full[torch.as_tensor(idx, device=full.device, dtype=torch.long)].cpu().numpy().view(np.uint32)
+    for r, i in enumerate(idx):
+        c = int(g["sample_nc"][r]) - 1
+        assert np.array_equal(rows[r, :c], g["sample_rows"][r, :c]), i
+    del full
+    torch.cuda.empty_cache()
+
+
 def test_uniform_1M_800_neighbors_full_size(ctx):
     """Full size (1M, ~800 neighbors, ngmax 1024, FP64 step): counts against a k-d tree and
     symmetric membership on sampled particles; density, IAD and momentum-energy of sampled

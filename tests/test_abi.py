@@ -177,6 +177,110 @@ void use(const cstone::OctreeNsView<double, uint64_t>& view, const cstone::Group
 """
 
 
+DROPIN_WITH_REFERENCE = r"""
+#include <utility>
+#include "cstone/findneighbors.hpp"
+#include "cstone/tree/csarray.hpp"
+#include "cstone/tree/octree.hpp"
+#include "cstone/tree/octree_gpu.h"
+#include "cstone/focus/source_center.hpp"
+#include "cstone/primitives/primitives_acc.hpp"
+#include "cstone/sfc/sfc_gpu.h"
+#include "cstone/traversal/groups.hpp"
+#include "sph/hydro_std/density.hpp"
+#include "sph/hydro_std/iad.hpp"
+#include "sph/hydro_std/momentum_energy.hpp"
+#include "sph/hydro_std/eos.hpp"
+#include "sph/hydro_ve/xmass.hpp"
+#include "sph/hydro_ve/ve_def_gradh.hpp"
+#include "sph/hydro_ve/eos.hpp"
+#include "sph/hydro_ve/iad_divv_curlv.hpp"
+#include "sph/hydro_ve/av_switches.hpp"
+#include "sph/hydro_ve/momentum_energy.hpp"
+#include "sph/find_neighbors.hpp"
+#include "sph/groups.hpp"
+#include "sph/update_h.hpp"
+#include "sph/positions.hpp"
+#include "sph/particles_data.hpp"
+#include "sphx_dropin.hpp"
+using Data = sphexa::ParticlesData<cstone::GpuTag>;
+void use(Data& d, cstone::GroupData<cstone::GpuTag>& groups, const cstone::OctreeNsView<double, uint64_t>& view,
+         cstone::OctreeView<uint64_t> ov, std::span<uint64_t> keys, std::span<unsigned> vals,
+         cstone::DeviceVector<uint64_t>& tree, cstone::DeviceVector<unsigned>& counts,
+         cstone::DeviceVector<uint64_t>& tmp, cstone::DeviceVector<int>& work, cstone::Vec3<double>* centers,
+         const double* x, unsigned* nb, unsigned* nc)
+{
+    cstone::Box<double> box(0, 1, cstone::BoundaryType::periodic);
+    cstone::computeSfcKeys<true>(x, x, x, keys.data(), keys.size(), box);
+    cstone::DeviceVector<uint64_t> kb;
+    cstone::DeviceVector<unsigned> vb;
+    cstone::sortByKey<true>(keys, vals, kb, vb, 1.05);
+    cstone::gatherAcc<true>(std::span<const unsigned>(vals), x, const_cast<double*>(x));
+    auto [t, c] = cstone::computeOctree(std::span<const uint64_t>(keys), 64);
+    (void)cstone::updateOctreeGpu(std::span<const uint64_t>(keys), 64, tree, counts, tmp, work);
+    cstone::buildOctreeGpu(t.data(), ov);
+    cstone::updateInternalTree(std::span<const uint64_t>(t), ov);
+    cstone::nodeFpCenters(std::span<const uint64_t>(ov.prefixes, 10), centers, centers, box);
+    cstone::findNeighbors(x, x, x, x, 0u, 10u, box, view, 150u, nb, nc);
+    sph::computeGroups(0, 10, d, box, groups);
+    sph::findNeighborsSfc(0, 10, d, box);
+    sph::computeDensity(groups.view(), d, box);
+    sph::computeEOS_HydroStd(0, 10, d);
+    sph::computeIAD(groups.view(), d, box);
+    sph::computeMomentumEnergySTD(groups.view(), d, box);
+    sph::computeXMass(groups.view(), d, box);
+    sph::computeVeDefGradh(groups.view(), d, box);
+    sph::computeEOS(0, 10, d);
+    sph::computeIadDivvCurlv(groups.view(), d, box);
+    sph::computeAVswitches(groups.view(), d, box);
+    sph::computeMomentumEnergy<false>(groups.view(), nullptr, d, box);
+    (void)sph::rhoTimestep(0, 10, d);
+    sph::computeTimestep(0, 10, d);
+    sph::computePositions(groups.view(), d, box, 1e-3f, util::array<float, 4>{});
+    (void)sph::updateSmoothingLength(groups.view(), d);
+}
+"""
+
+# every entry point of sphx_dropin.hpp, by the C ABI function it must reach
+DROPIN_CALLS = ("sphx_compute_sfc_keys", "sphx_sort_by_key", "sphx_gather_f64", "sphx_compute_octree",
+                "sphx_build_octree", "sphx_node_centers", "sphx_find_neighbors", "sphx_compute_groups",
+                "sphx_find_neighbors_sph", "sphx_compute_density", "sphx_compute_eos", "sphx_compute_iad",
+                "sphx_compute_momentum_energy", "sphx_compute_xmass", "sphx_compute_ve_def_gradh",
+                "sphx_compute_eos_ve", "sphx_compute_iad_divv_curlv", "sphx_compute_av_switches",
+                "sphx_compute_momentum_energy_ve", "sphx_rho_timestep", "sphx_compute_timestep",
+                "sphx_compute_positions", "sphx_update_smoothing_length")
+
+
+def test_dropin_header_overrides_reference_entry_points():
+    """include/sphx_dropin.hpp included after the reference's own headers: every call with the
+    reference's own types and argument lists (cstone::..., sph::..., ParticlesData<GpuTag>,
+    DeviceVector, OctreeNsView, GroupData) resolves, unambiguously, to the drop-in overload: the
+    object file calls the library's C ABI for each, and instantiates none of the reference's
+    implementations (no GPU kernels' host stubs, no CPU search, no MPI reduction).  The
+    reference's sph/ts_global.hpp needs mpi.h, which this image lacks, so computeTimestep and
+    rhoTimestep meet only the drop-in's declarations here."""
+    ref = "/root/reference"
+    if not os.path.isdir(os.path.join(ref, "domain", "include")):
+        pytest.skip("reference sources not present")
+    with tempfile.TemporaryDirectory() as d:
+        f, obj = os.path.join(d, "t.cpp"), os.path.join(d, "t.o")
+        open(f, "w").write(DROPIN_WITH_REFERENCE)
+        cmd = ["g++", "-std=c++20", "-c", "-Wall", "-Wno-unused-variable", "-Wno-unknown-pragmas", "-D__HIP_PLATFORM_AMD__",
+               "-I/opt/rocm/include", "-I", os.path.join(ROOT, "include"), "-I", f"{ref}/domain/include", "-I",
+               f"{ref}/sph/include", "-I", f"{ref}/main/src", f, "-o", obj]
+        out = subprocess.run(cmd, capture_output=True, text=True)
+        assert out.returncode == 0, out.stderr[-3000:]
+        undef = subprocess.run(["nm", "-u", obj], check=True, capture_output=True, text=True).stdout.split()
+        defined = subprocess.run(["nm", "-C", "--defined-only", obj], check=True, capture_output=True,
+                                 text=True).stdout
+    missing = [c for c in DROPIN_CALLS if c not in undef]
+    assert not missing, f"calls that did not reach the library: {missing}"
+    assert not [s for s in undef if "Gpu" in s or s.startswith("MPI_")], undef
+    for impl in ("findNeighborsSph", "computeDensityImpl", "computeIADImpl", "computeMomentumEnergyStdImpl",
+                 "updateOctree(", "buildOctreeCpu", "computeXMassImpl"):
+        assert impl not in defined, impl
+
+
 def _syntax_check(src, extra_includes=()):
     with tempfile.TemporaryDirectory() as d:
         f = os.path.join(d, "t.cpp")

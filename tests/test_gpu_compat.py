@@ -1,12 +1,19 @@
-"""The C++ drop-in shims at run time: tests/compat_driver.cpp (built by build()) runs one
-std-hydro time step through include/sphx_compat.hpp with its own device vectors, box, octree
-and group view types, the way HydroProp::computeForces and ::integrate call the reference.
+"""The C++ drop-in at run time: tests/compat_driver.cpp (built by build()) runs one std-hydro
+time step through include/sphx_dropin.hpp, calling the reference's entry points by their own
+names and argument lists (cstone::computeSfcKeys<true>, sortByKey<true>, gatherAcc<true>,
+updateOctreeGpu, buildOctreeGpu, nodeFpCenters, sph::computeGroups, findNeighborsSfc, the
+loops, computeTimestep, computePositions, updateSmoothingLength) with its own device vectors,
+box, octree views, group data and dataset, the way Domain::sync and HydroProp::computeForces
+and ::integrate call them; then the CPU-signature entry points on host memory
+(computeOctree, updateInternalTree, nodeFpCenters, findNeighbors).
 Its results are checked against the oracle on the same input:
 * sorted positions, neighbor counts: bit-exact;
 * rho <= 1e-13 relative; p, c <= 1e-13 relative; ax, ay, az, du <= 1e-10 of the field's
   largest magnitude (FP64 with FMA contraction, a different summation order);
 * Courant and new time step, positions, velocities, u and h after integrate <= 1e-10 relative
-  to the field's largest magnitude (they inherit the accelerations' rounding).
+  to the field's largest magnitude (they inherit the accelerations' rounding);
+* computeOctree on host keys equals updateOctreeGpu's tree; the plain findNeighbors counts over
+  host copies of the tree, and the lists of the first 64 particles: bit-exact.
 """
 import os
 import subprocess
@@ -52,6 +59,12 @@ def run_driver(inp, box):
     for k in ("x1", "y1", "z1", "vx1", "u1", "h1"):
         out[k] = np.frombuffer(raw, np.float64, n, o)
         o += 8 * n
+    out["same_tree"] = int(np.frombuffer(raw, np.int32, 1, o)[0])
+    o += 4
+    out["plain_nc"] = np.frombuffer(raw, np.uint32, n, o)
+    o += 4 * n
+    rows = min(64, n)
+    out["plain_nb"] = np.frombuffer(raw, np.uint32, rows * 150, o).reshape(rows, 150)
     return out
 
 
@@ -90,3 +103,10 @@ def test_compat_driver_time_step(case):
     for g, w in (("x1", "x"), ("y1", "y"), ("z1", "z"), ("vx1", "vx"), ("u1", "u")):
         assert scale(got[g], w1[w]) <= 1e-10, g
     assert rel_err(got["h1"], h1) <= 1e-12
+    # the CPU-signature entry points on host memory
+    assert got["same_tree"] == 1
+    pnb, pnc = CPU.find_neighbors(s["x"], s["y"], s["z"], want["h"], box, want["tree"], 150)
+    assert np.array_equal(got["plain_nc"], pnc)
+    for i in range(64):
+        c = min(int(pnc[i]), 150)
+        assert np.array_equal(got["plain_nb"][i, :c], pnb[i, :c]), i

@@ -355,9 +355,11 @@ struct SortWork
 
 // own == nullptr: buffers from the context's scratch, and digit passes that are constant
 // across all keys are detected (one read of the keys and one 8-byte copy to the host) and
-// skipped.  own != nullptr: the caller's buffers, every pass of keyBits runs, and the sort
-// is stream-ordered throughout (no host synchronisation, no memset, no timing events).
-// Both give the same result: a pass over a constant digit is the identity permutation.
+// skipped (measured: skipping the check for the ~1M-entry tree sorts costs more in extra
+// passes than the host round trip).  own != nullptr: the caller's buffers, every pass of
+// keyBits runs, and the sort is stream-ordered throughout (no host synchronisation, no
+// memset, no timing events).  Both give the same result: a pass over a constant digit is
+// the identity permutation.
 static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits,
                          const SortWork* own)
 {
@@ -374,11 +376,15 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     }
     else
     {
-        kAlt           = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
-        vAlt           = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
-        hist           = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
+        kAlt = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
+        vAlt = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
+        hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
+        if (!kAlt || (vals && !vAlt) || !hist) return SPHX_ERR_NOMEM;
+    }
+    if (!own)
+    {
         uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
-        if (!kAlt || (vals && !vAlt) || !hist || !vary) return SPHX_ERR_NOMEM;
+        if (!vary) return SPHX_ERR_NOMEM;
 
         // find the digit passes that change anything (one read of the keys + one 8-byte D2H)
         SPHX_HIP(ctx, hipMemsetAsync(vary, 0, sizeof(uint64_t), ctx->stream));

@@ -405,14 +405,24 @@ class ParticlesData:
         s = Particles()
         s.n, s.precision, s.ng0, s.ngmax, s.Kcour, s.K, s.gamma = (self.n, self.precision, self.ng0, self.ngmax,
                                                                    self.Kcour, self.K, self.gamma)
-        fields = ("x", "y", "z", "nc", "wh") + self.HYDRO + self.INTEGRATION
-        spec = {f: (_F64 if f in ("x", "y", "z") else (_I32 if f == "nc" else self.dtype),
-                    _lib.TABLE_SIZE if f == "wh" else self.n) for f in fields}
-        for f in fields:  # layout of every field first, then the device
-            _dev(getattr(self, f), *spec[f], f"ParticlesData.{f}", optional=True, on_device=False)
-        for f in fields:
-            p = _dev(getattr(self, f), *spec[f], f"ParticlesData.{f}", optional=True)
-            setattr(s, f, p.value if p is not None else None)
+        key = (self.precision, self.n)
+        if getattr(self, "_spec_key", None) != key:
+            fields = ("x", "y", "z", "nc", "wh") + self.HYDRO + self.INTEGRATION
+            self._spec = tuple((f, _F64 if f in ("x", "y", "z") else (_I32 if f == "nc" else self.dtype),
+                                _lib.TABLE_SIZE if f == "wh" else self.n) for f in fields)
+            self._spec_key = key
+        # one pass over the fields; the careful pass below only runs to name what is wrong
+        for f, want, n in self._spec:
+            t = getattr(self, f)
+            if t is None:
+                setattr(s, f, None)
+            elif t.dtype is want and t.is_cuda and t.numel() >= n and t.is_contiguous():
+                setattr(s, f, t.data_ptr())
+            else:
+                for g, w, m in self._spec:  # layout of every field first, then the device
+                    _dev(getattr(self, g), w, m, f"ParticlesData.{g}", optional=True, on_device=False)
+                for g, w, m in self._spec:
+                    _dev(getattr(self, g), w, m, f"ParticlesData.{g}", optional=True)
         s.minDtCourant = self.minDtCourant
         return s
 

@@ -204,6 +204,15 @@ int sphx_gather_f64(sphx_ctx* ctx, const uint32_t* order, const double* in, doub
 int sphx_gather_f32(sphx_ctx* ctx, const uint32_t* order, const float* in, float* out, uint64_t n);
 int sphx_gather_u32(sphx_ctx* ctx, const uint32_t* order, const uint32_t* in, uint32_t* out, uint64_t n);
 
+/* cstone::gatherArrays (domain/include/cstone/domain/layout.hpp:231-260) with the output
+ * offset 0: out[k][i] = in[k][order[i]] for count <= SPHX_GATHER_MAX_ARRAYS arrays of
+ * 4- or 8-byte elements (elemBytes[k]) in one launch.  in, out and elemBytes are host
+ * arrays of device pointers; outputs may overlap neither inputs nor each other (the
+ * reference gathers into scratch buffers and swaps them in). */
+#define SPHX_GATHER_MAX_ARRAYS 16
+int sphx_gather_arrays(sphx_ctx* ctx, const uint32_t* order, uint64_t n, int32_t count, const void* const* in,
+                       void* const* out, const uint32_t* elemBytes);
+
 /* ---- cornerstone octree --------------------------------------------------- */
 /* cstone::computeOctree (domain/include/cstone/tree/csarray.hpp:430-440; GPU loop
  * updateOctreeGpu, tree/update_gpu.cuh:41-64): the converged cornerstone leaf array

@@ -24,6 +24,7 @@ OPT_LOOP_SHAPE_FP64 = 3
 OPT_LOOP_SHAPE_MIXED = 4
 STAT_CANDIDATE_TESTS = 1
 STAT_SCRATCH_BYTES = 2
+GATHER_MAX_ARRAYS = 16
 TABLE_SIZE = 20000
 MAX_LEVEL = 21
 
@@ -87,6 +88,7 @@ SIGNATURES = {
     "sphx_gather_f64": (C.c_int, [_V, _V, _V, _V, _U64]),
     "sphx_gather_f32": (C.c_int, [_V, _V, _V, _V, _U64]),
     "sphx_gather_u32": (C.c_int, [_V, _V, _V, _V, _U64]),
+    "sphx_gather_arrays": (C.c_int, [_V, _V, _U64, C.c_int32, _V, _V, _V]),
     "sphx_compute_octree": (C.c_int, [_V, _V, _U64, _U32, _U32, C.POINTER(_I32), C.POINTER(_V), C.POINTER(_V),
                                       C.POINTER(_V)]),
     "sphx_octree_sizes": (C.c_int, [_I32, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),

@@ -320,18 +320,35 @@ __global__ __launch_bounds__(SORT_THREADS) void radixScatter(const uint64_t* __r
     }
 }
 
-// one pass over the keys: which digit passes are not constant
+// one pass over the keys: which digit passes are not constant.  Bit b varies iff the OR
+// and the AND of all keys differ in it; each block reduces both in registers and LDS and
+// enters them with one atomic each (acc[0] |= OR, acc[1] &= AND; acc starts {0, ~0}).
 __global__ __launch_bounds__(256) void radixDigitVariation(const uint64_t* __restrict__ keys, uint64_t n,
-                                                           uint64_t* __restrict__ orDiff)
+                                                           unsigned long long* __restrict__ acc)
 {
-    // OR over (key ^ key0): bit b set iff bit b varies
-    uint64_t k0  = keys[0];
-    uint64_t acc = 0;
+    __shared__ uint64_t part[2][4];
+    uint64_t            o = 0, a = ~uint64_t(0);
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        acc |= keys[i] ^ k0;
-    for (int o = 32; o > 0; o >>= 1)
-        acc |= __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0 && acc) atomicOr((unsigned long long*)orDiff, (unsigned long long)acc);
+    {
+        const uint64_t k = keys[i];
+        o |= k, a &= k;
+    }
+    for (int s = 32; s > 0; s >>= 1)
+        o |= __shfl_xor(o, s, 64), a &= __shfl_xor(a, s, 64);
+    if ((threadIdx.x & 63) == 0) part[0][threadIdx.x >> 6] = o, part[1][threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        o = part[0][0] | part[0][1] | part[0][2] | part[0][3];
+        a = part[1][0] & part[1][1] & part[1][2] & part[1][3];
+        atomicOr(acc, (unsigned long long)o);
+        atomicAnd(acc + 1, (unsigned long long)a);
+    }
+}
+
+__global__ void initDigitVariation(unsigned long long* acc)
+{
+    if (threadIdx.x == 0) acc[0] = 0ull, acc[1] = ~0ull;
 }
 
 __global__ void copyKV(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, uint64_t n)
@@ -354,7 +371,7 @@ struct SortWork
 };
 
 // own == nullptr: buffers from the context's scratch, and digit passes that are constant
-// across all keys are detected (one read of the keys and one 8-byte copy to the host) and
+// across all keys are detected (one read of the keys and one 16-byte copy to the host) and
 // skipped (measured: skipping the check for the ~1M-entry tree sorts costs more in extra
 // passes than the host round trip).  own != nullptr: the caller's buffers, every pass of
 // keyBits runs, and the sort is stream-ordered throughout (no host synchronisation, no
@@ -386,13 +403,17 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
         uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
         if (!vary) return SPHX_ERR_NOMEM;
 
-        // find the digit passes that change anything (one read of the keys + one 8-byte D2H)
-        SPHX_HIP(ctx, hipMemsetAsync(vary, 0, sizeof(uint64_t), ctx->stream));
-        unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8);
-        radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, vary);
+        // find the digit passes that change anything (one read of the keys + one 16-byte D2H)
+        auto* acc = reinterpret_cast<unsigned long long*>(vary);
+        initDigitVariation<<<1, 64, 0, ctx->stream>>>(acc);
+        SPHX_LAUNCH_CHECK(ctx, "initDigitVariation");
+        unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 4);
+        radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, acc);
         SPHX_LAUNCH_CHECK(ctx, "radixDigitVariation");
-        SPHX_HIP(ctx, hipMemcpyAsync(&varies, vary, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long oa[2] = {0, 0};
+        SPHX_HIP(ctx, hipMemcpyAsync(oa, acc, sizeof(oa), hipMemcpyDeviceToHost, ctx->stream));
         SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        varies = uint64_t(oa[0] ^ oa[1]);
     }
     if (!kAlt || (vals && !vAlt) || !hist) return argError(ctx, "sort: missing work buffer");
     sphx_ctx* tctx = own ? nullptr : ctx; // KTimer records nothing for the caller-buffer variant
@@ -500,6 +521,36 @@ __global__ __launch_bounds__(256) void gatherKernel(const uint32_t* __restrict__
     if (i < n) out[i] = in[order[i]];
 }
 
+// several arrays permuted by one ordering: order[i] is read once, every array's element
+// is loaded before any is stored (the loads overlap in flight)
+struct GatherSet
+{
+    const void* in[SPHX_GATHER_MAX_ARRAYS];
+    void*       out[SPHX_GATHER_MAX_ARRAYS];
+    uint32_t    wide; // bit k: array k has 8-byte elements, else 4-byte
+    int         count;
+};
+
+__global__ __launch_bounds__(256) void gatherArraysKernel(const uint32_t* __restrict__ order, GatherSet s, uint64_t n)
+{
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = order[i];
+    uint64_t       v[SPHX_GATHER_MAX_ARRAYS];
+#pragma unroll
+    for (int k = 0; k < SPHX_GATHER_MAX_ARRAYS; ++k)
+        if (k < s.count)
+            v[k] = ((s.wide >> k) & 1u) ? static_cast<const uint64_t*>(s.in[k])[j]
+                                        : uint64_t(static_cast<const uint32_t*>(s.in[k])[j]);
+#pragma unroll
+    for (int k = 0; k < SPHX_GATHER_MAX_ARRAYS; ++k)
+        if (k < s.count)
+        {
+            if ((s.wide >> k) & 1u) static_cast<uint64_t*>(s.out[k])[i] = v[k];
+            else static_cast<uint32_t*>(s.out[k])[i] = uint32_t(v[k]);
+        }
+}
+
 __global__ __launch_bounds__(256) void sequenceKernel(uint32_t* v, uint64_t n, uint32_t first)
 {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -544,6 +595,50 @@ int gatherT(sphx_ctx* ctx, const uint32_t* order, const T* in, T* out, uint64_t 
         gatherKernel<T><<<gridFor(n, 256), 256, 0, ctx->stream>>>(order, in, out, n);
     }
     SPHX_LAUNCH_CHECK(ctx, "gatherKernel");
+    return SPHX_OK;
+}
+
+static int gatherArrays(sphx_ctx* ctx, const uint32_t* order, uint64_t n, int32_t count, const void* const* in,
+                        void* const* out, const uint32_t* elemBytes)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (count < 0 || count > SPHX_GATHER_MAX_ARRAYS)
+        return fail(ctx, SPHX_ERR_ARG, "gather_arrays: count must be in [0, " + std::to_string(SPHX_GATHER_MAX_ARRAYS) + "]");
+    if (n == 0 || count == 0) return SPHX_OK;
+    if (!order || !in || !out || !elemBytes) return argError(ctx, "gather_arrays: null pointer");
+    GatherSet s{};
+    s.count = count;
+    for (int k = 0; k < count; ++k)
+    {
+        if (!in[k] || !out[k]) return fail(ctx, SPHX_ERR_ARG, "gather_arrays: null array " + std::to_string(k));
+        if (elemBytes[k] != 4 && elemBytes[k] != 8)
+            return fail(ctx, SPHX_ERR_ARG, "gather_arrays: element size of array " + std::to_string(k) + " must be 4 or 8");
+        s.in[k]  = in[k];
+        s.out[k] = out[k];
+        if (elemBytes[k] == 8) s.wide |= 1u << k;
+    }
+    // every destination is written while all sources are read: no destination may overlap
+    // a source or another destination
+    auto span = [&](const void* p, int k) {
+        auto b = reinterpret_cast<uintptr_t>(p);
+        return std::pair<uintptr_t, uintptr_t>(b, b + n * elemBytes[k]);
+    };
+    for (int a = 0; a < count; ++a)
+        for (int b = 0; b < count; ++b)
+        {
+            auto o = span(out[a], a), i = span(in[b], b), o2 = span(out[b], b);
+            if (o.first < i.second && i.first < o.second)
+                return fail(ctx, SPHX_ERR_ARG, "gather_arrays: output " + std::to_string(a) + " overlaps input " +
+                                         std::to_string(b));
+            if (a != b && o.first < o2.second && o2.first < o.second)
+                return fail(ctx, SPHX_ERR_ARG, "gather_arrays: outputs " + std::to_string(a) + " and " + std::to_string(b) +
+                                         " overlap");
+        }
+    {
+        KTimer kt_(ctx, "gatherArraysKernel");
+        gatherArraysKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(order, s, n);
+    }
+    SPHX_LAUNCH_CHECK(ctx, "gatherArraysKernel");
     return SPHX_OK;
 }
 
@@ -773,6 +868,12 @@ int sphx_gather_f32(sphx_ctx* ctx, const uint32_t* order, const float* in, float
 {
     return gatherT(ctx, order, in, out, n);
 }
+int sphx_gather_arrays(sphx_ctx* ctx, const uint32_t* order, uint64_t n, int32_t count, const void* const* in,
+                       void* const* out, const uint32_t* elemBytes)
+{
+    return gatherArrays(ctx, order, n, count, in, out, elemBytes);
+}
+
 int sphx_gather_u32(sphx_ctx* ctx, const uint32_t* order, const uint32_t* in, uint32_t* out, uint64_t n)
 {
     return gatherT(ctx, order, in, out, n);

@@ -702,32 +702,46 @@ __global__ __launch_bounds__(256) void nsExport(GroupsDev G, SphLists L, uint32_
 // groups_gpu.cuh:37-77,148-216).  Fixed 64-particle groups are split after particle p
 // when |X(p+1) - X(p)|^2 > min(distCrit^2, (2 h_p / minExtent)^2) in box-length units,
 // distCrit = tolFactor * cbrt(volume of the smallest leaf holding a particle of the group).
+//
+// One wave per fixed group.  The leaves holding the group's particles p0..p1 are the
+// non-empty leaves from leaf(p0) (the last leaf with layout <= p0; layout[0] = 0) through
+// leaf(p1): the wave finds leaf(p0) with a 64-way search (four dependent probes for 2^24
+// leaves) and then walks the leaves after it 64 at a time until one starts beyond p1,
+// which for a group of 64 particles is nearly always one step.
 template<class Th>
-__global__ __launch_bounds__(64) void grpSplitMasks(uint32_t first, uint32_t last, const double* __restrict__ x,
-                                                    const double* __restrict__ y, const double* __restrict__ z,
-                                                    const Th* __restrict__ h, const uint64_t* __restrict__ leaves,
-                                                    int numLeaves, const uint32_t* __restrict__ layout, DBox box,
-                                                    float tolFactor, uint64_t* __restrict__ masks,
-                                                    uint32_t* __restrict__ numSegments)
+__global__ __launch_bounds__(256) void grpSplitMasks(uint32_t first, uint32_t last, uint32_t numFixed,
+                                                     const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const Th* __restrict__ h,
+                                                     const uint64_t* __restrict__ leaves, int numLeaves,
+                                                     const uint32_t* __restrict__ layout, DBox box, float tolFactor,
+                                                     uint64_t* __restrict__ masks, uint32_t* __restrict__ numSegments)
 {
 #pragma clang fp contract(off)
-    const uint32_t g    = blockIdx.x;
-    const unsigned lane = threadIdx.x;
-    uint32_t       bi   = min(first + 64u * g + lane, last - 1);
-    // leaf containing bi: last leaf with layout <= bi
-    int lo = 0, cnt = numLeaves;
-    while (cnt > 0)
+    const uint32_t g    = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63u;
+    if (g >= numFixed) return; // whole waves
+    const uint32_t p0 = first + 64u * g;
+    const uint32_t p1 = min(p0 + 63u, last - 1);
+    const uint32_t bi = min(p0 + lane, p1);
+
+    int lo = 0, hi = numLeaves; // layout[lo] <= p0; leaf(p0) in [lo, hi)
+    while (hi - lo > 1)
     {
-        int step = cnt / 2, mid = lo + step;
-        if (layout[mid] <= bi)
-        {
-            lo = mid + 1;
-            cnt -= step + 1;
-        }
-        else { cnt = step; }
+        const int  stride = (hi - lo + 63) / 64;
+        const int  m      = lo + int(lane) * stride;
+        const bool le     = m < hi && layout[m] <= p0;
+        const int  k      = 63 - int(clz64(ballot(le))); // lane 0 always holds
+        hi                = min(hi, lo + (k + 1) * stride);
+        lo                = lo + k * stride;
     }
-    int      leaf  = lo - 1;
-    unsigned level = treeLevel(leaves[leaf + 1] - leaves[leaf]);
+    unsigned level = 0;
+    for (int c = lo;; c += 64)
+    {
+        const int  m  = c + int(lane);
+        const bool in = m < numLeaves && layout[m] <= p1; // m <= leaf(p1)
+        if (in && layout[m + 1] > layout[m]) level = max(level, treeLevel(leaves[m + 1] - leaves[m]));
+        if (ballot(in) != ~uint64_t(0)) break;
+    }
     // unit-box leaf volume 2^(-3 level); its cube root 2^(-level) is exact
     int      maxLevel = int(waveMax(int(level)));
     double   distCrit = ldexp(1.0, -maxLevel) * double(tolFactor);
@@ -904,14 +918,20 @@ int sphx_compute_groups(sphx_ctx* ctx, uint32_t first, uint32_t last, const doub
     uint32_t* segs  = scratchT<uint32_t>(ctx, S_NS_RCOUNT, size_t(numFixed) + 1);
     if (!masks || !segs) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
-    if (precision == SPHX_FP64)
-        grpSplitMasks<double><<<numFixed, 64, 0, ctx->stream>>>(first, last, x, y, z, static_cast<const double*>(h),
-                                                                tree->leaves, tree->numLeafNodes, tree->layout, db,
-                                                                tolFactor, masks, segs);
-    else
-        grpSplitMasks<float><<<numFixed, 64, 0, ctx->stream>>>(first, last, x, y, z, static_cast<const float*>(h),
-                                                               tree->leaves, tree->numLeafNodes, tree->layout, db,
-                                                               tolFactor, masks, segs);
+    {
+        KTimer   kt_(ctx, "grpSplitMasks");
+        uint32_t grid = (numFixed + 3u) / 4u;
+        if (precision == SPHX_FP64)
+            grpSplitMasks<double><<<grid, 256, 0, ctx->stream>>>(first, last, numFixed, x, y, z,
+                                                                 static_cast<const double*>(h), tree->leaves,
+                                                                 tree->numLeafNodes, tree->layout, db, tolFactor,
+                                                                 masks, segs);
+        else
+            grpSplitMasks<float><<<grid, 256, 0, ctx->stream>>>(first, last, numFixed, x, y, z,
+                                                                static_cast<const float*>(h), tree->leaves,
+                                                                tree->numLeafNodes, tree->layout, db, tolFactor,
+                                                                masks, segs);
+    }
     SPHX_LAUNCH_CHECK(ctx, "grpSplitMasks");
     SPHX_TRY(exclusiveScanU32(ctx, segs, segs, numFixed));
     uint32_t numGroups = 0;

@@ -9,6 +9,7 @@ this module                    reference entry point
 ``compute_sfc_keys``           cstone::computeSfcKeys<true> (domain/include/cstone/sfc/sfc_gpu.h:26)
 ``sort_by_key``                cstone::sortByKey<true> (primitives/primitives_acc.hpp:115)
 ``gather``                     cstone::gather (primitives/gather.hpp:90)
+``gather_arrays``              cstone::gatherArrays (domain/layout.hpp:231)
 ``compute_octree``             cstone::computeOctree (tree/csarray.hpp:430)
 ``build_octree``               cstone::updateInternalTree / buildOctreeGpu (tree/octree.hpp:361)
 ``node_fp_centers``            cstone::nodeFpCenters (focus/source_center.hpp:146)
@@ -235,6 +236,29 @@ def gather(ctx: Context, order, src, dst=None):
     ctx.check(fns[src.dtype](ctx.h, _dev(order, _I32, n, "order"), _dev(src, src.dtype, n, "src"),
                              _dev(dst, src.dtype, n, "dst"), n))
     return dst
+
+
+_GATHER_BYTES = {torch.float64: 8, torch.int64: 8, torch.float32: 4, torch.int32: 4}
+
+
+def gather_arrays(ctx: Context, order, pairs):
+    """dst[i] = src[order[i]] for every (src, dst) in ``pairs`` (at most 16, all of
+    order's length, 4- or 8-byte dtypes, dst of src's dtype), in one launch."""
+    pairs = list(pairs)
+    if not pairs:
+        return
+    if len(pairs) > _lib.GATHER_MAX_ARRAYS:
+        raise ValueError(f"gather_arrays: {len(pairs)} arrays, at most {_lib.GATHER_MAX_ARRAYS}")
+    n = order.numel()
+    k = len(pairs)
+    src_p, dst_p, size = (C.c_void_p * k)(), (C.c_void_p * k)(), (C.c_uint32 * k)()
+    for j, (src, dst) in enumerate(pairs):
+        if src.dtype not in _GATHER_BYTES:
+            raise TypeError(f"gather_arrays: array {j} has dtype {src.dtype}, expected a 4- or 8-byte type")
+        src_p[j] = _dev(src, src.dtype, n, f"src {j}")
+        dst_p[j] = _dev(dst, src.dtype, n, f"dst {j}")
+        size[j] = _GATHER_BYTES[src.dtype]
+    ctx.check(ctx.lib.sphx_gather_arrays(ctx.h, _dev(order, _I32, n, "order"), n, k, src_p, dst_p, size))
 
 
 # ------------------------------------------------------------------------------------------- octree
@@ -757,9 +781,9 @@ class HydroStep:
         ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
         sort_by_key(ctx, self.keys, self.order)
         mark(3)
-        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION:
-            if f in inp:
-                gather(ctx, self.order, inp[f], getattr(d, f))
+        gather_arrays(ctx, self.order, [(inp[f], getattr(d, f)) for f in
+                                         ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION
+                                         if f in inp])
         mark(4)
         leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(5)
@@ -846,11 +870,11 @@ class VeHydroStep(HydroStep):
         ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
         sort_by_key(ctx, self.keys, self.order)
         mark(3)
-        for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION:
-            if f in inp:
-                gather(ctx, self.order, inp[f], getattr(d, f))
+        pairs = [(inp[f], getattr(d, f)) for f in ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") +
+                 ParticlesData.INTEGRATION if f in inp]
         if "alpha" in inp:
-            gather(ctx, self.order, inp["alpha"], self.ve.alpha)
+            pairs.append((inp["alpha"], self.ve.alpha))
+        gather_arrays(ctx, self.order, pairs)
         mark(4)
         leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(5)

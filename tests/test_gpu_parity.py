@@ -73,6 +73,45 @@ def test_keys_skip_remove_key(ctx):
     assert np.array_equal(u64(k), ref)
 
 
+# ----------------------------------------------------------------------------- gather
+@pytest.mark.parametrize("n", [1, 1000, 300007])
+def test_gather_arrays_equal_take(ctx, n):
+    """gatherArrays: every array permuted by one ordering in one launch, bit-exact with
+    numpy's take, for 4- and 8-byte arrays (16 at once, the maximum)."""
+    from sphexa_amd import hydro
+
+    print("case gather_arrays", n, flush=True)
+    rng = np.random.default_rng(n)
+    order = rng.permutation(n).astype(np.int32)
+    srcs = []
+    for k in range(16):
+        dt = (np.float64, np.float32, np.int32, np.int64)[k % 4]
+        srcs.append(rng.standard_normal(n).astype(dt) if k % 4 < 2 else rng.integers(-2**30, 2**30, n).astype(dt))
+    ds = [dev(a) for a in srcs]
+    outs = [torch.empty_like(t) for t in ds]
+    hydro.gather_arrays(ctx, dev(order), list(zip(ds, outs)))
+    for a, o in zip(srcs, outs):
+        assert np.array_equal(o.cpu().numpy(), a[order])
+    one = torch.empty_like(ds[0])
+    hydro.gather_arrays(ctx, dev(order), [(ds[0], one)])
+    assert np.array_equal(one.cpu().numpy(), hydro.gather(ctx, dev(order), ds[0]).cpu().numpy())
+
+
+def test_gather_arrays_rejects_overlap_and_count(ctx):
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import SphxError
+
+    order = dev(np.arange(64, dtype=np.int32))
+    a, b = torch.zeros(64, dtype=torch.float64, device="cuda"), torch.zeros(64, dtype=torch.float64, device="cuda")
+    with pytest.raises(SphxError, match="overlaps input"):
+        hydro.gather_arrays(ctx, order, [(a, b), (b, torch.empty_like(b))])
+    big = torch.zeros(128, dtype=torch.float64, device="cuda")
+    with pytest.raises(SphxError, match="outputs 0 and 1 overlap"):
+        hydro.gather_arrays(ctx, order, [(a, big[:64]), (b, big[32:96])])
+    with pytest.raises(ValueError, match="at most 16"):
+        hydro.gather_arrays(ctx, order, [(a, b)] * 17)
+
+
 # ----------------------------------------------------------------------------- sort
 @pytest.mark.parametrize("n,kind", [(1, "rand"), (100, "rand"), (4096, "rand"), (4097, "dup"), (150001, "rand"),
                                     (70000, "dup"), (30000, "lowbits")])
@@ -331,14 +370,23 @@ def test_export_neighbors_cpu_layout(ctx):
 
 
 # ----------------------------------------------------------------------------- target groups
-@pytest.mark.parametrize("case", ["uniform", "sedov", "clustered"])
+@pytest.mark.parametrize("case", ["uniform", "sedov", "clustered", "fine"])
 @pytest.mark.parametrize("hdtype", ["f64", "f32", "none"])
 def test_compute_groups_bit_exact(ctx, case, hdtype):
-    """computeGroupSplits on the GPU against the oracle: identical boundaries."""
+    """computeGroupSplits on the GPU against the oracle: identical boundaries.  "fine" uses
+    one-particle leaves on clumped particles, so that one 64-particle group spans hundreds
+    of leaves, most of them empty."""
     from sphexa_amd import hydro
 
     print("case groups", case, hdtype, flush=True)
-    if case == "sedov":
+    bucket = 64
+    if case == "fine":
+        inp, box = uniform_box(4000, seed=54, periodic=False)
+        rng = np.random.default_rng(55)
+        for a in ("x", "y", "z"):
+            inp[a][:2000] = 0.3 + 1e-4 * rng.random(2000)
+        bucket = 1
+    elif case == "sedov":
         inp, box = sedov_lattice(24)
     elif case == "uniform":
         inp, box = uniform_box(30000, seed=51)
@@ -350,9 +398,13 @@ def test_compute_groups_bit_exact(ctx, case, hdtype):
             sl = slice(k * 6000, (k + 1) * 6000)
             for a in ("x", "y", "z"):
                 inp[a][sl] = c + 0.02 * rng.random(6000)
-    keys, s, t = _gpu_tree(ctx, inp, box)
-    rt = CPU.tree(keys, box)
+    keys, s, t = _gpu_tree(ctx, inp, box, bucket)
+    rt = CPU.tree(keys, box, bucket)
     n = len(s["x"])
+    if case == "fine":
+        spans = np.searchsorted(rt["layout"][:-1], np.arange(63, n, 64), side="right") - \
+            np.searchsorted(rt["layout"][:-1], np.arange(0, n - 63, 64), side="right")
+        assert spans.max() > 128  # the leaf walk takes several steps
     if hdtype == "none":
         h_dev, h_ref = None, None
     elif hdtype == "f32":

@@ -24,6 +24,36 @@ int failHip(sphx_ctx* ctx, hipError_t e, const char* what)
 
 int argError(sphx_ctx* ctx, const char* what) { return fail(ctx, SPHX_ERR_ARG, what); }
 
+int readBackStart(sphx_ctx* ctx, const void* src, size_t bytes)
+{
+    if (!ctx->hostStage || !ctx->readEvent || bytes > kHostStageBytes)
+        return fail(ctx, SPHX_ERR_STATE, "readBackStart: no pinned staging for this read");
+    SPHX_HIP(ctx, hipMemcpyAsync(ctx->hostStage, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipEventRecord(ctx->readEvent, ctx->stream));
+    return SPHX_OK;
+}
+
+int readBackFinish(sphx_ctx* ctx, void* dst, size_t bytes)
+{
+    SPHX_HIP(ctx, hipEventSynchronize(ctx->readEvent));
+    std::memcpy(dst, ctx->hostStage, bytes);
+    return SPHX_OK;
+}
+
+int readBack(sphx_ctx* ctx, const void* src, void* dst, size_t bytes)
+{
+    if (ctx->hostStage && bytes <= kHostStageBytes)
+    {
+        SPHX_HIP(ctx, hipMemcpyAsync(ctx->hostStage, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::memcpy(dst, ctx->hostStage, bytes);
+        return SPHX_OK;
+    }
+    SPHX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SPHX_OK;
+}
+
 void* scratch(sphx_ctx* ctx, Slot s, size_t bytes)
 {
     DevBuf& b = ctx->buf[s];
@@ -203,25 +233,61 @@ constexpr int SORT_ITEMS   = SPHX_SORT_ITEMS; // keys per thread of a tile
 constexpr int SORT_TILE    = SORT_THREADS * SORT_ITEMS; // 4096
 constexpr int RADIX        = 256;
 
+// tileVar != nullptr: the tile's OR and AND of its keys go to tileVar[2 tile], [2 tile + 1]
 __global__ __launch_bounds__(SORT_THREADS) void radixUpsweep(const uint64_t* __restrict__ keys, uint64_t n,
                                                              int shift, uint32_t* __restrict__ tileHist,
-                                                             uint32_t numTiles)
+                                                             uint32_t numTiles, uint64_t* __restrict__ tileVar)
 {
     __shared__ uint32_t h[4][RADIX];
+    __shared__ uint64_t part[2][4];
     unsigned wave = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < 4 * RADIX; d += SORT_THREADS)
         (&h[0][0])[d] = 0;
     __syncthreads();
     uint64_t base = uint64_t(blockIdx.x) * SORT_TILE;
+    uint64_t o = 0, a = ~uint64_t(0);
 #pragma unroll 4
     for (int k = 0; k < SORT_ITEMS; ++k)
     {
         uint64_t i = base + uint64_t(k) * SORT_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[wave][unsigned(keys[i] >> shift) & 255u], 1u);
+        if (i < n)
+        {
+            const uint64_t key = keys[i];
+            o |= key, a &= key;
+            atomicAdd(&h[wave][unsigned(key >> shift) & 255u], 1u);
+        }
+    }
+    if (tileVar)
+    {
+        for (int s = 32; s > 0; s >>= 1)
+            o |= __shfl_xor(o, s, 64), a &= __shfl_xor(a, s, 64);
+        if ((threadIdx.x & 63u) == 0) part[0][wave] = o, part[1][wave] = a;
     }
     __syncthreads();
     unsigned d = threadIdx.x;
     tileHist[uint64_t(d) * numTiles + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    if (tileVar && threadIdx.x == 0)
+    {
+        tileVar[2 * uint64_t(blockIdx.x)]     = part[0][0] | part[0][1] | part[0][2] | part[0][3];
+        tileVar[2 * uint64_t(blockIdx.x) + 1] = part[1][0] & part[1][1] & part[1][2] & part[1][3];
+    }
+}
+
+// bits that vary across all keys: OR of the tiles' ORs ^ AND of their ANDs (one block)
+__global__ __launch_bounds__(256) void radixVaryingBits(const uint64_t* __restrict__ tileVar, uint32_t numTiles,
+                                                         uint64_t* __restrict__ varies)
+{
+    __shared__ uint64_t part[2][4];
+    uint64_t            o = 0, a = ~uint64_t(0);
+    for (uint32_t t = threadIdx.x; t < numTiles; t += 256)
+        o |= tileVar[2 * uint64_t(t)], a &= tileVar[2 * uint64_t(t) + 1];
+    for (int s = 32; s > 0; s >>= 1)
+        o |= __shfl_xor(o, s, 64), a &= __shfl_xor(a, s, 64);
+    if ((threadIdx.x & 63u) == 0) part[0][threadIdx.x >> 6] = o, part[1][threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        varies[0] = (part[0][0] | part[0][1] | part[0][2] | part[0][3]) ^
+                    (part[1][0] & part[1][1] & part[1][2] & part[1][3]);
 }
 
 template<bool HasVals>
@@ -320,37 +386,6 @@ __global__ __launch_bounds__(SORT_THREADS) void radixScatter(const uint64_t* __r
     }
 }
 
-// one pass over the keys: which digit passes are not constant.  Bit b varies iff the OR
-// and the AND of all keys differ in it; each block reduces both in registers and LDS and
-// enters them with one atomic each (acc[0] |= OR, acc[1] &= AND; acc starts {0, ~0}).
-__global__ __launch_bounds__(256) void radixDigitVariation(const uint64_t* __restrict__ keys, uint64_t n,
-                                                           unsigned long long* __restrict__ acc)
-{
-    __shared__ uint64_t part[2][4];
-    uint64_t            o = 0, a = ~uint64_t(0);
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-    {
-        const uint64_t k = keys[i];
-        o |= k, a &= k;
-    }
-    for (int s = 32; s > 0; s >>= 1)
-        o |= __shfl_xor(o, s, 64), a &= __shfl_xor(a, s, 64);
-    if ((threadIdx.x & 63) == 0) part[0][threadIdx.x >> 6] = o, part[1][threadIdx.x >> 6] = a;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        o = part[0][0] | part[0][1] | part[0][2] | part[0][3];
-        a = part[1][0] & part[1][1] & part[1][2] & part[1][3];
-        atomicOr(acc, (unsigned long long)o);
-        atomicAnd(acc + 1, (unsigned long long)a);
-    }
-}
-
-__global__ void initDigitVariation(unsigned long long* acc)
-{
-    if (threadIdx.x == 0) acc[0] = 0ull, acc[1] = ~0ull;
-}
-
 __global__ void copyKV(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, uint64_t n)
 {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
@@ -370,13 +405,14 @@ struct SortWork
     uint32_t* scanSums[3];
 };
 
-// own == nullptr: buffers from the context's scratch, and digit passes that are constant
-// across all keys are detected (one read of the keys and one 16-byte copy to the host) and
-// skipped (measured: skipping the check for the ~1M-entry tree sorts costs more in extra
-// passes than the host round trip).  own != nullptr: the caller's buffers, every pass of
-// keyBits runs, and the sort is stream-ordered throughout (no host synchronisation, no
-// memset, no timing events).  Both give the same result: a pass over a constant digit is
-// the identity permutation.
+// own == nullptr: buffers from the context's scratch, and the passes over digits that are
+// constant across all keys are skipped: the first pass's histogram kernel also forms each
+// tile's OR and AND of the keys, one block reduces them to the bits that vary, and the
+// host reads those 8 bytes while the first pass's scan and scatter run (measured: skipping
+// no pass costs the ~2M-entry tree sorts more in extra passes than the read).  own !=
+// nullptr: the caller's buffers, every pass of keyBits runs, and the sort is stream-ordered
+// throughout (no host synchronisation, no memset, no timing events).  Both give the same
+// result: a pass over a constant digit is the identity permutation.
 static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits,
                          const SortWork* own)
 {
@@ -398,22 +434,12 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
         hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
         if (!kAlt || (vals && !vAlt) || !hist) return SPHX_ERR_NOMEM;
     }
-    if (!own)
+    uint64_t *tileVar = nullptr, *vary = nullptr;
+    if (!own && keyBits > 8)
     {
-        uint64_t* vary = scratchT<uint64_t>(ctx, S_SMALL, 8);
-        if (!vary) return SPHX_ERR_NOMEM;
-
-        // find the digit passes that change anything (one read of the keys + one 16-byte D2H)
-        auto* acc = reinterpret_cast<unsigned long long*>(vary);
-        initDigitVariation<<<1, 64, 0, ctx->stream>>>(acc);
-        SPHX_LAUNCH_CHECK(ctx, "initDigitVariation");
-        unsigned vgrid = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 4);
-        radixDigitVariation<<<vgrid, 256, 0, ctx->stream>>>(keys, n, acc);
-        SPHX_LAUNCH_CHECK(ctx, "radixDigitVariation");
-        unsigned long long oa[2] = {0, 0};
-        SPHX_HIP(ctx, hipMemcpyAsync(oa, acc, sizeof(oa), hipMemcpyDeviceToHost, ctx->stream));
-        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        varies = uint64_t(oa[0] ^ oa[1]);
+        tileVar = scratchT<uint64_t>(ctx, S_SORT_VAR, 2 * uint64_t(numTiles));
+        vary    = scratchT<uint64_t>(ctx, S_SMALL, 8);
+        if (!tileVar || !vary) return SPHX_ERR_NOMEM;
     }
     if (!kAlt || (vals && !vAlt) || !hist) return argError(ctx, "sort: missing work buffer");
     sphx_ctx* tctx = own ? nullptr : ctx; // KTimer records nothing for the caller-buffer variant
@@ -422,14 +448,30 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     uint32_t* vSrc = vals;
     uint64_t* kDst = kAlt;
     uint32_t* vDst = vAlt;
+    bool pending = false; // the varying bits are being read
     for (int shift = 0; shift < keyBits; shift += 8)
     {
+        if (pending)
+        {
+            SPHX_TRY(ctx->hostStage ? readBackFinish(ctx, &varies, sizeof(varies))
+                                    : readBack(ctx, vary, &varies, sizeof(varies)));
+            pending = false;
+        }
         if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
+        const bool first = shift == 0 && tileVar;
         {
             KTimer kt_(tctx, "radixUpsweep");
-            radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles);
+            radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles,
+                                                                     first ? tileVar : nullptr);
         }
         SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
+        if (first)
+        {
+            radixVaryingBits<<<1, 256, 0, ctx->stream>>>(tileVar, numTiles, vary);
+            SPHX_LAUNCH_CHECK(ctx, "radixVaryingBits");
+            if (ctx->hostStage) SPHX_TRY(readBackStart(ctx, vary, sizeof(uint64_t)));
+            pending = true;
+        }
         SPHX_TRY(scanLevel(ctx, hist, hist, uint64_t(RADIX) * numTiles, 0, own ? own->scanSums : nullptr));
         if (vals)
             {
@@ -736,6 +778,19 @@ int sphx_ctx_create(int device, void* stream, sphx_ctx** out)
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(stream);
     ctx->noallocScratch = noallocScratchOf(device);
+    // pinned staging for small reads; without it the reads go to pageable memory
+    if (hipHostMalloc(&ctx->hostStage, kHostStageBytes, hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        ctx->hostStage = nullptr;
+    }
+    if (ctx->hostStage && hipEventCreateWithFlags(&ctx->readEvent, hipEventDisableTiming) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        (void)hipHostFree(ctx->hostStage);
+        ctx->hostStage = nullptr;
+        ctx->readEvent = nullptr;
+    }
     int cu      = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
         ctx->numCU = cu;
@@ -787,6 +842,8 @@ int sphx_ctx_destroy(sphx_ctx* ctx)
         (void)hipEventDestroy(e);
     for (auto& b : ctx->buf)
         if (b.p) (void)hipFree(b.p);
+    if (ctx->hostStage) (void)hipHostFree(ctx->hostStage);
+    if (ctx->readEvent) (void)hipEventDestroy(ctx->readEvent);
     delete ctx;
     return SPHX_OK;
 }
@@ -900,8 +957,7 @@ int sphx_make_global_box(sphx_ctx* ctx, const double* x, const double* y, const 
         boxExtremaFinal<<<1, 64, 0, ctx->stream>>>(partial, blocks, partial + size_t(blocks) * 6);
         SPHX_LAUNCH_CHECK(ctx, "boxExtremaFinal");
         double e[6];
-        SPHX_HIP(ctx, hipMemcpyAsync(e, partial + size_t(blocks) * 6, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
-        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SPHX_TRY(readBack(ctx, partial + size_t(blocks) * 6, e, sizeof(e)));
         if (!keep[0]) r.xmin = e[0], r.xmax = e[3];
         if (!keep[1]) r.ymin = e[1], r.ymax = e[4];
         if (!keep[2]) r.zmin = e[2], r.zmax = e[5];

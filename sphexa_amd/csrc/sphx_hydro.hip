@@ -252,8 +252,7 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt)
                               : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt));
     unsigned long long bits = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
-    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
     double v;
     std::memcpy(&v, &bits, sizeof(v));
     d->minDtCourant = v;
@@ -413,8 +412,7 @@ int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sp
     SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt)
                               : fast::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt));
     unsigned long long bits = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&bits, dt, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
-    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
     double v;
     std::memcpy(&v, &bits, sizeof(v));
     d->minDtCourant = v;

@@ -212,8 +212,7 @@ int sphx_compute_timestep(sphx_ctx* ctx, uint32_t first, uint32_t last, const sp
             }
             SPHX_LAUNCH_CHECK(ctx, "minH2A2Kernel");
             unsigned long long bits = 0;
-            SPHX_HIP(ctx, hipMemcpyAsync(&bits, mb, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
-            SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            SPHX_TRY(readBack(ctx, mb, &bits, sizeof(bits)));
             double m;
             std::memcpy(&m, &bits, sizeof(m));
             minDtAcc = ts->etaAcc * std::pow(m, 0.25);
@@ -300,8 +299,7 @@ int sphx_update_smoothing_length(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     }
     SPHX_LAUNCH_CHECK(ctx, "updateHKernel");
     uint32_t r = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&r, status, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SPHX_TRY(readBack(ctx, status, &r, sizeof(r)));
     if (keysRemoved) *keysRemoved = (r & 1u) ? 1 : 0;
     if (r & 2u)
         return fail(ctx, SPHX_ERR_ARG, "update_smoothing_length: a neighbor count exceeds the factor table (" +

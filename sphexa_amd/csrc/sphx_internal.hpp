@@ -47,6 +47,7 @@ enum Slot : int
     S_TABLE_D,       // kernel-derivative table (VE gradh loop), FP64
     S_HY_REC,        // packed neighbor-side records of the SPH loops
     S_HFAC,          // smoothing-length update factors per neighbor count (FP64, then FP32)
+    S_SORT_VAR,      // radix sort: per-tile OR and AND of the keys (first pass)
     S_NUM_SLOTS
 };
 
@@ -193,12 +194,25 @@ struct sphx_ctx
     long long hfacNg0 = -1;
     // this device's address of the allocation-free sort's work array (nullptr: unavailable)
     void* noallocScratch = nullptr;
+    // pinned host staging for small device-to-host reads (kHostStageBytes; nullptr: pageable
+    // reads) and the event that marks a started read
+    void*      hostStage = nullptr;
+    hipEvent_t readEvent = nullptr;
 };
 
 namespace sphx
 {
 
 int fail(sphx_ctx* ctx, int code, const std::string& msg);
+
+// Small device-to-host reads through the context's pinned staging buffer.  readBack copies
+// bytes from src and waits for the stream; readBackStart enqueues the copy and marks it
+// with an event, and readBackFinish waits for that copy alone (work enqueued after the
+// start keeps running) and delivers the bytes.  One started read at a time.
+constexpr size_t kHostStageBytes = 4096;
+int              readBack(sphx_ctx* ctx, const void* src, void* dst, size_t bytes);
+int              readBackStart(sphx_ctx* ctx, const void* src, size_t bytes);
+int              readBackFinish(sphx_ctx* ctx, void* dst, size_t bytes);
 int failHip(sphx_ctx* ctx, hipError_t e, const char* what);
 int argError(sphx_ctx* ctx, const char* what);
 

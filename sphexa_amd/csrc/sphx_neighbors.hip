@@ -845,8 +845,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
                                                                                                 flags);
         SPHX_LAUNCH_CHECK(ctx, "nsSumTests");
         NsFlags hf;
-        SPHX_HIP(ctx, hipMemcpyAsync(&hf, flags, sizeof(NsFlags), hipMemcpyDeviceToHost, ctx->stream));
-        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SPHX_TRY(readBack(ctx, flags, &hf, sizeof(NsFlags)));
         ctx->nsTests += double(hf.tests);
         if (hf.overflow) return fail(ctx, SPHX_ERR_STACK, "neighbor search: traversal capacity exceeded");
         if (hf.badGroup) return argError(ctx, "neighbor search: a target group holds more than 64 particles");
@@ -935,8 +934,7 @@ int sphx_compute_groups(sphx_ctx* ctx, uint32_t first, uint32_t last, const doub
     SPHX_LAUNCH_CHECK(ctx, "grpSplitMasks");
     SPHX_TRY(exclusiveScanU32(ctx, segs, segs, numFixed));
     uint32_t numGroups = 0;
-    SPHX_HIP(ctx, hipMemcpyAsync(&numGroups, segs + numFixed, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SPHX_TRY(readBack(ctx, segs + numFixed, &numGroups, sizeof(uint32_t)));
     uint32_t* bounds = scratchT<uint32_t>(ctx, S_GROUPS, size_t(numGroups) + 1);
     if (!bounds) return SPHX_ERR_NOMEM;
     grpBoundaries<<<gridFor(numFixed, 256), 256, 0, ctx->stream>>>(first, last, numFixed, masks, segs, bounds);

@@ -244,8 +244,7 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
         }
         SPHX_LAUNCH_CHECK(ctx, "csCountSplits");
         SPHX_TRY(exclusiveScanU32(ctx, sOfs, sOfs, nn));
-        SPHX_HIP(ctx, hipMemcpyAsync(&S, sOfs + nn, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        SPHX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SPHX_TRY(readBack(ctx, sOfs + nn, &S, sizeof(uint32_t)));
     }
     uint64_t L = uint64_t(S) * 7 + 1;
     if (L >= (uint64_t(1) << 31)) return argError(ctx, "compute_octree: too many leaves");

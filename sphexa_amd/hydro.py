@@ -40,6 +40,7 @@ torch's current stream.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import math
 from dataclasses import dataclass, field
@@ -151,6 +152,16 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.sphx_ctx_synchronize(self.h))
+
+    @contextlib.contextmanager
+    def on_stream(self, stream):
+        """Launch on ``stream`` (a default-priority torch stream of this device) inside the
+        block, then on the context's own stream again."""
+        self.check(self.lib.sphx_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+        try:
+            yield
+        finally:
+            self.check(self.lib.sphx_ctx_set_stream(self.h, C.c_void_p(self.stream.cuda_stream)))
 
     def close(self):
         if self.h:
@@ -755,6 +766,24 @@ class HydroStep:
         self.octree = None
         self.events = None
         self.box_calls = 0
+        self.side = torch.cuda.Stream(device=ctx.device)  # default priority
+
+    def gather_inputs(self, pairs, overlap: bool):
+        """The permutation of the inputs (gatherArrays).  overlap: on a second stream, so
+        that it runs beside the tree build (which reads only the keys); the returned
+        callable makes the context's stream wait for it before the fields are read."""
+        ctx = self.ctx
+        if not overlap:
+            gather_arrays(ctx, self.order, pairs)
+            return lambda: None
+        sorted_ = torch.cuda.Event()
+        sorted_.record(ctx.stream)
+        self.side.wait_event(sorted_)
+        with ctx.on_stream(self.side):
+            gather_arrays(ctx, self.order, pairs)
+        gathered = torch.cuda.Event()
+        gathered.record(self.side)
+        return lambda: ctx.stream.wait_event(gathered)
 
     def sync_box(self, inp: dict) -> Box:
         """The box of this call, as Domain::sync on one rank (domain/assignment.hpp:102-106):
@@ -781,9 +810,9 @@ class HydroStep:
         ctx.check(ctx.lib.sphx_sequence(ctx.h, _ptr(self.order), self.n, 0))
         sort_by_key(ctx, self.keys, self.order)
         mark(3)
-        gather_arrays(ctx, self.order, [(inp[f], getattr(d, f)) for f in
-                                         ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION
-                                         if f in inp])
+        join = self.gather_inputs([(inp[f], getattr(d, f)) for f in
+                                   ("x", "y", "z", "h", "m", "vx", "vy", "vz", "u") + ParticlesData.INTEGRATION
+                                   if f in inp], overlap=not timed)
         mark(4)
         leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(5)
@@ -792,6 +821,7 @@ class HydroStep:
         mark(6)
         node_fp_centers(ctx, o, box)
         self.octree = o
+        join()
         mark(7)
         g = compute_groups(ctx, d.x, d.y, d.z, d.h, o, box) if self.spatial_groups else None
         self.groups = g
@@ -874,7 +904,7 @@ class VeHydroStep(HydroStep):
                  ParticlesData.INTEGRATION if f in inp]
         if "alpha" in inp:
             pairs.append((inp["alpha"], self.ve.alpha))
-        gather_arrays(ctx, self.order, pairs)
+        join = self.gather_inputs(pairs, overlap=not timed)
         mark(4)
         leaves, counts, layout = compute_octree(ctx, self.keys, self.bucket)
         mark(5)
@@ -883,6 +913,7 @@ class VeHydroStep(HydroStep):
         mark(6)
         node_fp_centers(ctx, o, box)
         self.octree = o
+        join()
         mark(7)
         g = compute_groups(ctx, d.x, d.y, d.z, d.h, o, box) if self.spatial_groups else None
         self.groups = g

@@ -602,27 +602,63 @@ __global__ __launch_bounds__(256) void sequenceKernel(uint32_t* v, uint64_t n, u
 // ===========================================================================
 // Hilbert keys: sfc3D (domain/include/cstone/sfc/sfc.hpp:141-178) with separately
 // rounded products, as in the reference expression int(floor(x*mx) - xmin*mx)
+// The keys are encoded two levels per lookup: every workgroup first tabulates, in LDS,
+// hilbertStep applied twice for each of the 48 states and each pair of levels' six raw
+// bits (digits and next state), and the top level from the initial state; a key is then
+// one top-level lookup and ten pair lookups (grid-stride over the particles).
 __global__ __launch_bounds__(256) void sfcKeysKernel(const double* __restrict__ x, const double* __restrict__ y,
                                                      const double* __restrict__ z, uint64_t* __restrict__ keys,
                                                      uint64_t n, DBox box)
 {
 #pragma clang fp contract(off)
-    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (keys[i] == kRemoveKey) return;
+    static_assert(kMaxLevel == 21, "one top level and ten pairs of levels");
+    __shared__ uint16_t pairTab[48 * 64]; // [state][x1x0 y1y0 z1z0] = digits (6 bits) | next state << 6
+    __shared__ uint16_t topTab[8];        // [x y z] from state 0 = digit | next state << 3
+    for (uint32_t e = threadIdx.x; e < 48u * 64u; e += 256u)
+    {
+        uint32_t       s  = e >> 6;
+        const uint32_t xb = (e >> 4) & 3u, yb = (e >> 2) & 3u, zb = e & 3u;
+        const uint32_t d1 = hilbertStep(s, ((xb >> 1) << 2) | ((yb >> 1) << 1) | (zb >> 1));
+        const uint32_t d0 = hilbertStep(s, ((xb & 1u) << 2) | ((yb & 1u) << 1) | (zb & 1u));
+        pairTab[e]        = uint16_t((d1 << 3) | d0 | (s << 6));
+    }
+    if (threadIdx.x < 8)
+    {
+        uint32_t       s = 0;
+        const uint32_t d = hilbertStep(s, threadIdx.x);
+        topTab[threadIdx.x] = uint16_t(d | (s << 3));
+    }
+    __syncthreads();
+
     const double cube   = double(1u << kMaxLevel);
     const int    mcoord = (1 << kMaxLevel) - 1;
-    double       mx = cube * box.ilen[0], my = cube * box.ilen[1], mz = cube * box.ilen[2];
-    int          ix = int(floor(x[i] * mx) - box.lo[0] * mx);
-    int          iy = int(floor(y[i] * my) - box.lo[1] * my);
-    int          iz = int(floor(z[i] * mz) - box.lo[2] * mz);
-    ix = min(ix, mcoord);
-    iy = min(iy, mcoord);
-    iz = min(iz, mcoord);
-    ix = max(ix, 0); // coordinates below the box floor would be undefined in the reference
-    iy = max(iy, 0);
-    iz = max(iz, 0);
-    keys[i] = hilbertKey(unsigned(ix), unsigned(iy), unsigned(iz));
+    const double mx = cube * box.ilen[0], my = cube * box.ilen[1], mz = cube * box.ilen[2];
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256)
+    {
+        if (keys[i] == kRemoveKey) continue;
+        int ix = int(floor(x[i] * mx) - box.lo[0] * mx);
+        int iy = int(floor(y[i] * my) - box.lo[1] * my);
+        int iz = int(floor(z[i] * mz) - box.lo[2] * mz);
+        ix = min(ix, mcoord);
+        iy = min(iy, mcoord);
+        iz = min(iz, mcoord);
+        ix = max(ix, 0); // coordinates below the box floor would be undefined in the reference
+        iy = max(iy, 0);
+        iz = max(iz, 0);
+        const uint32_t ux = uint32_t(ix), uy = uint32_t(iy), uz = uint32_t(iz);
+        const uint32_t t  = topTab[((ux >> 20) << 2) | ((uy >> 20) << 1) | (uz >> 20)];
+        uint64_t       key = t & 7u;
+        uint32_t       s   = t >> 3;
+#pragma unroll
+        for (int l = 18; l >= 0; l -= 2)
+        {
+            const uint32_t idx = (((ux >> l) & 3u) << 4) | (((uy >> l) & 3u) << 2) | ((uz >> l) & 3u);
+            const uint32_t e   = pairTab[(s << 6) | idx];
+            key                = (key << 6) | (e & 63u);
+            s                  = e >> 6;
+        }
+        keys[i] = key;
+    }
 }
 
 template<class T>
@@ -871,7 +907,8 @@ int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const
     if (!validBox(box)) return argError(ctx, "compute_sfc_keys: invalid box");
     {
         KTimer kt_(ctx, "sfcKeysKernel");
-        sfcKeysKernel<<<gridFor(n, 256), 256, 0, ctx->stream>>>(x, y, z, keys, n, makeDBox(box));
+        const unsigned g = std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 4);
+        sfcKeysKernel<<<g, 256, 0, ctx->stream>>>(x, y, z, keys, n, makeDBox(box));
     }
     SPHX_LAUNCH_CHECK(ctx, "sfcKeysKernel");
     return SPHX_OK;

@@ -346,6 +346,52 @@ __host__ __device__ inline uint64_t hilbertKey(unsigned px, unsigned py, unsigne
     return key;
 }
 
+// hilbertKey as a state machine.  Encoding a level reflects and permutes the remaining
+// coordinate bits, so before each level the coordinates are seen through a state =
+// (which axis feeds each of the three slots, which slots are reflected): 6 x 8 = 48
+// states, numbered 8 * permutation + reflections; the initial state 0 is the identity.
+// hilbertStep encodes one level from the state and the level's raw bits
+// (x << 2 | y << 1 | z): it returns the output digit and advances the state, the same
+// updates hilbertKey applies to px, py, pz.
+__host__ __device__ inline uint32_t hilbertStep(uint32_t& state, uint32_t raw)
+{
+    constexpr uint32_t m2h = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) | (6u << 15) | (4u << 18) |
+                             (5u << 21);
+    // slot axes of the 6 permutations, 2 bits per slot (slot 0 lowest), 6 bits per permutation
+    constexpr uint64_t perms = uint64_t(0u | 1u << 2 | 2u << 4) | uint64_t(0u | 2u << 2 | 1u << 4) << 6 |
+                               uint64_t(1u | 0u << 2 | 2u << 4) << 12 | uint64_t(1u | 2u << 2 | 0u << 4) << 18 |
+                               uint64_t(2u | 0u << 2 | 1u << 4) << 24 | uint64_t(2u | 1u << 2 | 0u << 4) << 30;
+    const uint32_t pa = uint32_t(perms >> (6u * (state >> 3))) & 63u;
+    uint32_t       a0 = pa & 3u, a1 = (pa >> 2) & 3u, a2 = (pa >> 4) & 3u;
+    uint32_t       f0 = state & 1u, f1 = (state >> 1) & 1u, f2 = (state >> 2) & 1u;
+    const uint32_t xi = ((raw >> (2u - a0)) & 1u) ^ f0;
+    const uint32_t yi = ((raw >> (2u - a1)) & 1u) ^ f1;
+    const uint32_t zi = ((raw >> (2u - a2)) & 1u) ^ f2;
+    const uint32_t digit = (m2h >> (3u * ((xi << 2) | (yi << 1) | zi))) & 7u;
+    f0 ^= xi & ((!yi) | zi);
+    f1 ^= (xi & (yi | zi)) | (yi & (!zi));
+    f2 ^= (xi & (!yi) & (!zi)) | (yi & (!zi));
+    if (zi)
+    {
+        const uint32_t ta = a0, tf = f0;
+        a0 = a1, f0 = f1;
+        a1 = a2, f1 = f2;
+        a2 = ta, f2 = tf;
+    }
+    else if (!yi)
+    {
+        const uint32_t ta = a0, tf = f0;
+        a0 = a2, f0 = f2;
+        a2 = ta, f2 = tf;
+    }
+    const uint32_t code = a0 | a1 << 2 | a2 << 4;
+    uint32_t       p    = 0;
+    for (uint32_t k = 1; k < 6; ++k)
+        if ((uint32_t(perms >> (6u * k)) & 63u) == code) p = k;
+    state = 8u * p + (f0 | f1 << 1 | f2 << 2);
+    return digit;
+}
+
 __host__ __device__ inline void hilbertDecode(uint64_t key, unsigned& ox, unsigned& oy, unsigned& oz)
 {
     unsigned px = 0, py = 0, pz = 0;

@@ -139,16 +139,20 @@ __device__ void groupRangesOf(uint32_t g, GroupsDev G, const double* __restrict_
         int nxt    = cur ^ 1;
         for (int b = 0; b < fcount; b += 64)
         {
-            int  idx   = b + int(lane);
-            int  node  = idx < fcount ? front[cur][idx] : -1;
+            int idx  = b + int(lane);
+            int node = idx < fcount ? front[cur][idx] : -1;
+            // the node's child offset and leaf index are loaded with its box (one memory
+            // round trip per level), whether or not the box overlaps
+            int  childOf = 0, leafOf = 0;
+            if (node >= 0) childOf = childOffsets[node], leafOf = internalToLeaf[node];
             bool ov    = node >= 0 && nodeOverlapsBox(centers, sizes, node, gc, gh, box);
-            int  child = ov ? childOffsets[node] : 0;
+            int  child = ov ? childOf : 0;
             bool leaf  = ov && child == 0;
             bool inner = ov && child != 0;
 
             uint64_t mL   = ballot(leaf);
             int      posL = lcount + int(popc64(mL & lt));
-            if (leaf && posL < LCAP) leafList[posL] = internalToLeaf[node];
+            if (leaf && posL < LCAP) leafList[posL] = leafOf;
             lcount += int(popc64(mL));
 
             uint64_t mI   = ballot(inner);

@@ -414,7 +414,7 @@ struct SortWork
 // throughout (no host synchronisation, no memset, no timing events).  Both give the same
 // result: a pass over a constant digit is the identity permutation.
 static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits,
-                         const SortWork* own)
+                         const SortWork* own, int lowBits = 0)
 {
     if (n <= 1) return SPHX_OK;
     if (n >= (uint64_t(1) << 32)) return argError(ctx, "sort: n must be < 2^32");
@@ -435,7 +435,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
         if (!kAlt || (vals && !vAlt) || !hist) return SPHX_ERR_NOMEM;
     }
     uint64_t *tileVar = nullptr, *vary = nullptr;
-    if (!own && keyBits > 8)
+    if (!own && keyBits - 8 * (lowBits / 8) > 8)
     {
         tileVar = scratchT<uint64_t>(ctx, S_SORT_VAR, 2 * uint64_t(numTiles));
         vary    = scratchT<uint64_t>(ctx, S_SMALL, 8);
@@ -449,7 +449,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     uint64_t* kDst = kAlt;
     uint32_t* vDst = vAlt;
     bool pending = false; // the varying bits are being read
-    for (int shift = 0; shift < keyBits; shift += 8)
+    for (int shift = 8 * (lowBits / 8); shift < keyBits; shift += 8)
     {
         if (pending)
         {
@@ -458,7 +458,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
             pending = false;
         }
         if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
-        const bool first = shift == 0 && tileVar;
+        const bool first = shift == 8 * (lowBits / 8) && tileVar;
         {
             KTimer kt_(tctx, "radixUpsweep");
             radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles,
@@ -498,9 +498,9 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     return SPHX_OK;
 }
 
-int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits)
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits)
 {
-    return radixSortWith(ctx, keys, vals, n, keyBits, nullptr);
+    return radixSortWith(ctx, keys, vals, n, keyBits, nullptr, std::max(0, std::min(lowBits, keyBits)));
 }
 
 // ---------------------------------------------------------------------------

@@ -287,7 +287,9 @@ int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
 
 // stable LSD radix sort of 64-bit keys (+ optional 32-bit values) in place;
 // only the bits in [0, keyBits) are examined
-int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits);
+// lowBits: bits below it are known to be equal across all keys (passes start at the digit
+// holding bit lowBits)
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits = 0);
 
 // ---------------------------------------------------------------------------
 // key bit algebra (64-bit Hilbert keys, 21 levels, 63 key bits)

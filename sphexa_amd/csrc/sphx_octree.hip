@@ -36,14 +36,19 @@ __device__ inline void splitLevels(const uint64_t* keys, uint32_t n, uint32_t bu
     else { hi = -1; }
 }
 
+// numS[i]: the split nodes that start at particle i; *maxLevel: the deepest split level
+// (raised only where it grows, so that few of the waves issue the atomic)
 __global__ __launch_bounds__(256) void csCountSplits(const uint64_t* __restrict__ keys, uint32_t n, uint32_t bucket,
-                                                     uint32_t* __restrict__ numS)
+                                                     uint32_t* __restrict__ numS, uint32_t* __restrict__ maxLevel)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int lo, hi;
-    splitLevels(keys, n, bucket, i, lo, hi);
-    numS[i] = hi >= lo ? uint32_t(hi - lo + 1) : 0u;
+    int      lo = 0, hi = -1;
+    if (i < n) splitLevels(keys, n, bucket, i, lo, hi);
+    if (i < n) numS[i] = hi >= lo ? uint32_t(hi - lo + 1) : 0u;
+    const int top = int(waveMax(hi >= lo ? hi : -1));
+    if ((threadIdx.x & 63u) == 0 && top >= 0 &&
+        uint32_t(top) > __hip_atomic_load(maxLevel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(maxLevel, uint32_t(top));
 }
 
 __global__ __launch_bounds__(256) void csEmitChildren(const uint64_t* __restrict__ keys, uint32_t n, uint32_t bucket,
@@ -233,18 +238,23 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
     if (n >= (uint64_t(1) << 31)) return argError(ctx, "compute_octree: n must be < 2^31");
     uint32_t nn = uint32_t(n);
 
-    uint32_t S = 0;
-    uint32_t* sOfs = scratchT<uint32_t>(ctx, S_CS_NUMS, uint64_t(nn) + 1);
+    // sOfs[0..nn]: the scan of the split-node counts (sOfs[nn] = S); sOfs[nn + 1]: the
+    // deepest split level, read back with S
+    uint32_t  S = 0, maxSplit = 0;
+    uint32_t* sOfs = scratchT<uint32_t>(ctx, S_CS_NUMS, uint64_t(nn) + 2);
     if (!sOfs) return SPHX_ERR_NOMEM;
     if (nn > 0)
     {
+        SPHX_HIP(ctx, hipMemsetAsync(sOfs + nn + 1, 0, sizeof(uint32_t), ctx->stream));
         {
             KTimer kt_(ctx, "csCountSplits");
-            csCountSplits<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs);
+            csCountSplits<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, sOfs + nn + 1);
         }
         SPHX_LAUNCH_CHECK(ctx, "csCountSplits");
         SPHX_TRY(exclusiveScanU32(ctx, sOfs, sOfs, nn));
-        SPHX_TRY(readBack(ctx, sOfs + nn, &S, sizeof(uint32_t)));
+        uint32_t sm[2];
+        SPHX_TRY(readBack(ctx, sOfs + nn, sm, sizeof(sm)));
+        S = sm[0], maxSplit = sm[1];
     }
     uint64_t L = uint64_t(S) * 7 + 1;
     if (L >= (uint64_t(1) << 31)) return argError(ctx, "compute_octree: too many leaves");
@@ -262,7 +272,9 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
             csEmitChildren<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, child);
         }
         SPHX_LAUNCH_CHECK(ctx, "csEmitChildren");
-        SPHX_TRY(radixSortPairs(ctx, child, nullptr, L, 64));
+        // children of level-l nodes are multiples of the level-(l + 1) range 8^(20 - l): the
+        // low 3 (20 - maxSplit) bits of every child key are zero
+        SPHX_TRY(radixSortPairs(ctx, child, nullptr, L, 64, 3 * (int(kMaxLevel) - 1 - int(maxSplit))));
     }
     csFinishLeaves<<<1, 64, 0, ctx->stream>>>(child, uint32_t(L));
     SPHX_LAUNCH_CHECK(ctx, "csFinishLeaves");

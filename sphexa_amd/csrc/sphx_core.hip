@@ -192,10 +192,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void scanAddTileOffsets(uint32_t* out
 // tiles of one scan level over n values (n + 1 outputs)
 constexpr uint64_t scanTilesOf(uint64_t n) { return (n + 1 + SCAN_TILE - 1) / SCAN_TILE; }
 
-// levelSums: caller-provided tile-sum buffers per level (numTiles + 1 entries each), or
-// nullptr for the context's scratch slots
-static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, int level,
-                     uint32_t* const* levelSums)
+// the tile sums of each level in the context's scratch slots
+static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, int level)
 {
     uint64_t numTiles = scanTilesOf(n);
     if (numTiles == 1)
@@ -205,12 +203,12 @@ static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
         return SPHX_OK;
     }
     if (level > 2) return fail(ctx, SPHX_ERR_ARG, "scan too large");
-    uint32_t* sums = levelSums ? levelSums[level] : scratchT<uint32_t>(ctx, Slot(S_SCAN0 + level), numTiles + 1);
-    if (!sums) return levelSums ? fail(ctx, SPHX_ERR_ARG, "scan: no tile-sum buffer for this level") : SPHX_ERR_NOMEM;
+    uint32_t* sums = scratchT<uint32_t>(ctx, Slot(S_SCAN0 + level), numTiles + 1);
+    if (!sums) return SPHX_ERR_NOMEM;
     scanTiles<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(in, out, n, sums);
     SPHX_LAUNCH_CHECK(ctx, "scanTiles");
     // exclusive scan of the tile sums in place (numTiles entries, total discarded into sums[numTiles])
-    SPHX_TRY(scanLevel(ctx, sums, sums, numTiles, level + 1, levelSums));
+    SPHX_TRY(scanLevel(ctx, sums, sums, numTiles, level + 1));
     scanAddTileOffsets<<<unsigned(numTiles), SCAN_THREADS, 0, ctx->stream>>>(out, n, sums);
     SPHX_LAUNCH_CHECK(ctx, "scanAddTileOffsets");
     return SPHX_OK;
@@ -218,13 +216,15 @@ static int scanLevel(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
 
 int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n)
 {
-    return scanLevel(ctx, in, out, n, 0, nullptr);
+    return scanLevel(ctx, in, out, n, 0);
 }
 
 // ===========================================================================
-// LSD radix sort, 8-bit digits.  Per pass: tile histograms -> scan -> stable
-// scatter through an LDS-staged local sort so that global writes are
-// contiguous runs per digit.
+// LSD radix sort, 8-bit digits.  Per pass: tile histograms -> per-digit scan over the
+// tiles -> stable scatter through an LDS-staged local sort so that global writes are
+// contiguous runs per digit.  Three launches per pass: the digit-major histogram is
+// scanned one column (digit) per block, and each scatter tile adds the exclusive scan
+// of the 256 digit totals itself.
 constexpr int SORT_THREADS = 256;
 #ifndef SPHX_SORT_ITEMS
 #define SPHX_SORT_ITEMS 12 // measured: 16 -> 12 cut the 256^3 key sort from 2.1 to 1.9 ms
@@ -271,6 +271,37 @@ __global__ __launch_bounds__(SORT_THREADS) void radixUpsweep(const uint64_t* __r
         tileVar[2 * uint64_t(blockIdx.x)]     = part[0][0] | part[0][1] | part[0][2] | part[0][3];
         tileVar[2 * uint64_t(blockIdx.x) + 1] = part[1][0] & part[1][1] & part[1][2] & part[1][3];
     }
+}
+
+// Exclusive scan of each digit's column of tile counts, in place (hist[d * numTiles + t]),
+// and the digit's total into hist[RADIX * numTiles + d]; one block per digit.
+constexpr int COL_ITEMS = 8;
+__global__ __launch_bounds__(256) void radixColumnScan(uint32_t* __restrict__ hist, uint32_t numTiles)
+{
+    __shared__ uint32_t sWave[4];
+    uint32_t*           col   = hist + uint64_t(blockIdx.x) * numTiles;
+    uint32_t            carry = 0;
+    for (uint32_t base = 0; base < numTiles; base += 256u * COL_ITEMS)
+    {
+        const uint32_t b = base + threadIdx.x * COL_ITEMS;
+        uint32_t       v[COL_ITEMS], sum = 0;
+#pragma unroll
+        for (int k = 0; k < COL_ITEMS; ++k)
+        {
+            v[k] = b + k < numTiles ? col[b + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + blockExclusiveScan256(sum, sWave, total);
+#pragma unroll
+        for (int k = 0; k < COL_ITEMS; ++k)
+        {
+            if (b + k < numTiles) col[b + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) hist[uint64_t(RADIX) * numTiles + blockIdx.x] = carry;
 }
 
 // bits that vary across all keys: OR of the tiles' ORs ^ AND of their ANDs (one block)
@@ -354,10 +385,13 @@ __global__ __launch_bounds__(SORT_THREADS) void radixScatter(const uint64_t* __r
         wcount[1][d] = c0;
         wcount[2][d] = c0 + c1;
         wcount[3][d] = c0 + c1 + c2;
-        uint32_t total;
+        uint32_t total, all;
         uint32_t start = blockExclusiveScan256(tot, sWave, total);
+        // the digit's first output position: the totals of the smaller digits plus the
+        // counts of this digit in the tiles before this one
+        uint32_t below = blockExclusiveScan256(tileOfs[uint64_t(RADIX) * numTiles + d], sWave, all);
         dStart[d]      = start;
-        gOfs[d]        = tileOfs[uint64_t(d) * numTiles + blockIdx.x];
+        gOfs[d]        = below + tileOfs[uint64_t(d) * numTiles + blockIdx.x];
     }
     __syncthreads();
 
@@ -395,14 +429,13 @@ __global__ void copyKV(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uin
     }
 }
 
-// Work buffers of one sort: ping-pong keys and values (n each), the per-tile digit
-// histograms (RADIX * tiles + 1) and the tile sums of their scan per level.
+// Work buffers of one sort: ping-pong keys and values (n each) and the per-tile digit
+// histograms followed by the digit totals (RADIX * (tiles + 1)).
 struct SortWork
 {
     uint64_t* kAlt;
     uint32_t* vAlt;
     uint32_t* hist;
-    uint32_t* scanSums[3];
 };
 
 // own == nullptr: buffers from the context's scratch, and the passes over digits that are
@@ -431,7 +464,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     {
         kAlt = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
         vAlt = vals ? scratchT<uint32_t>(ctx, S_SORT_VALS, n) : nullptr;
-        hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * numTiles + 1);
+        hist = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * (numTiles + 1));
         if (!kAlt || (vals && !vAlt) || !hist) return SPHX_ERR_NOMEM;
     }
     uint64_t *tileVar = nullptr, *vary = nullptr;
@@ -472,7 +505,8 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
             if (ctx->hostStage) SPHX_TRY(readBackStart(ctx, vary, sizeof(uint64_t)));
             pending = true;
         }
-        SPHX_TRY(scanLevel(ctx, hist, hist, uint64_t(RADIX) * numTiles, 0, own ? own->scanSums : nullptr));
+        radixColumnScan<<<RADIX, 256, 0, ctx->stream>>>(hist, numTiles);
+        SPHX_LAUNCH_CHECK(ctx, "radixColumnScan");
         if (vals)
             {
                 KTimer kt_(tctx, "radixScatter");
@@ -513,17 +547,12 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
 // that overlap on one GPU share the array and race.
 constexpr uint64_t kNaMaxN  = uint64_t(1) << 21; // 2,097,152 keys (a 128^3 share)
 constexpr uint64_t kNaTiles = (kNaMaxN + SORT_TILE - 1) / SORT_TILE;
-constexpr uint64_t kNaScanN = uint64_t(RADIX) * kNaTiles; // histogram entries scanned
-constexpr uint64_t kNaT0 = scanTilesOf(kNaScanN), kNaT1 = scanTilesOf(kNaT0), kNaT2 = scanTilesOf(kNaT1);
-static_assert(scanTilesOf(kNaT2) == 1, "the histogram scan must fit three levels");
+constexpr uint64_t kNaHistN = uint64_t(RADIX) * (kNaTiles + 1); // tile histograms + digit totals
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-constexpr size_t kNaKeysOff  = 0;
-constexpr size_t kNaValsOff  = kNaKeysOff + align256(kNaMaxN * sizeof(uint64_t));
-constexpr size_t kNaHistOff  = kNaValsOff + align256(kNaMaxN * sizeof(uint32_t));
-constexpr size_t kNaScan0Off = kNaHistOff + align256((kNaScanN + 1) * sizeof(uint32_t));
-constexpr size_t kNaScan1Off = kNaScan0Off + align256((kNaT0 + 1) * sizeof(uint32_t));
-constexpr size_t kNaScan2Off = kNaScan1Off + align256((kNaT1 + 1) * sizeof(uint32_t));
-constexpr size_t kNaBytes    = kNaScan2Off + align256((kNaT2 + 1) * sizeof(uint32_t)); // ~24.6 MiB
+constexpr size_t kNaKeysOff = 0;
+constexpr size_t kNaValsOff = kNaKeysOff + align256(kNaMaxN * sizeof(uint64_t));
+constexpr size_t kNaHistOff = kNaValsOff + align256(kNaMaxN * sizeof(uint32_t));
+constexpr size_t kNaBytes   = kNaHistOff + align256(kNaHistN * sizeof(uint32_t)); // ~24.7 MiB
 
 namespace noalloc
 {
@@ -938,9 +967,6 @@ int sphx_sort_by_key_noalloc(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, ui
     w.kAlt        = reinterpret_cast<uint64_t*>(base + kNaKeysOff);
     w.vAlt        = reinterpret_cast<uint32_t*>(base + kNaValsOff);
     w.hist        = reinterpret_cast<uint32_t*>(base + kNaHistOff);
-    w.scanSums[0] = reinterpret_cast<uint32_t*>(base + kNaScan0Off);
-    w.scanSums[1] = reinterpret_cast<uint32_t*>(base + kNaScan1Off);
-    w.scanSums[2] = reinterpret_cast<uint32_t*>(base + kNaScan2Off);
     return radixSortWith(ctx, keys, values, n, 64, &w);
 }
 

@@ -115,9 +115,8 @@ __global__ __launch_bounds__(256) void positionsKernel(uint32_t first, uint32_t 
 
 // accelerationTimestep (sph/include/sph/ts_global.hpp:48-71): min_i h_i^2 / |a_i|^2,
 // h_i^2 formed in the field type as the reference's HydroType product; NaN terms are
-// skipped.  Non-negative doubles order like their bit patterns, so the per-wave
-// minimum enters *minBits with a 64-bit atomic min, issued only when it is smaller
-// than the value already stored.
+// skipped.  Non-negative doubles order like their bit patterns, so the block's
+// minimum enters *minBits with a 64-bit atomic min.
 template<class T>
 __global__ __launch_bounds__(256) void minH2A2Kernel(uint32_t first, uint32_t last, const T* __restrict__ h,
                                                      const T* __restrict__ ax, const T* __restrict__ ay,
@@ -138,10 +137,15 @@ __global__ __launch_bounds__(256) void minH2A2Kernel(uint32_t first, uint32_t la
         double other = __shfl_xor(m, o, 64);
         if (other < m) m = other;
     }
-    if ((threadIdx.x & 63u) == 0 && m < INFINITY)
+    // one atomic per block (a device-scope access per wave to one address serializes)
+    __shared__ double wmin[4];
+    if ((threadIdx.x & 63u) == 0) wmin[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        auto bits = (unsigned long long)__double_as_longlong(m);
-        if (bits < __hip_atomic_load(minBits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(minBits, bits);
+        for (int w = 1; w < 4; ++w)
+            if (wmin[w] < m) m = wmin[w];
+        if (m < INFINITY) atomicMin(minBits, (unsigned long long)__double_as_longlong(m));
     }
 }
 

@@ -21,7 +21,7 @@ enum Slot : int
 {
     S_SORT_KEYS = 0, // radix sort ping-pong keys
     S_SORT_VALS,     // radix sort ping-pong values
-    S_SORT_HIST,     // per-tile digit histograms (+ their scan)
+    S_SORT_HIST,     // per-tile digit histograms (+ the digit totals)
     S_SCAN0,         // device-wide scan block sums, level 0
     S_SCAN1,         // level 1
     S_SCAN2,         // level 2
@@ -48,6 +48,7 @@ enum Slot : int
     S_HY_REC,        // packed neighbor-side records of the SPH loops
     S_HFAC,          // smoothing-length update factors per neighbor count (FP64, then FP32)
     S_SORT_VAR,      // radix sort: per-tile OR and AND of the keys (first pass)
+    S_MINDT,         // SPH loops: per-wave time-step minima before their reduction
     S_NUM_SLOTS
 };
 

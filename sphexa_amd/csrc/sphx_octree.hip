@@ -36,19 +36,29 @@ __device__ inline void splitLevels(const uint64_t* keys, uint32_t n, uint32_t bu
     else { hi = -1; }
 }
 
-// numS[i]: the split nodes that start at particle i; *maxLevel: the deepest split level
-// (raised only where it grows, so that few of the waves issue the atomic)
+// numS[i]: the split nodes that start at particle i; *maxLevel: the deepest split level.
+// Grid-stride over the particles with one atomic per block: a device-scope access per
+// wave to the one address serializes (it made this pass 9x slower than its key reads).
 __global__ __launch_bounds__(256) void csCountSplits(const uint64_t* __restrict__ keys, uint32_t n, uint32_t bucket,
                                                      uint32_t* __restrict__ numS, uint32_t* __restrict__ maxLevel)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    int      lo = 0, hi = -1;
-    if (i < n) splitLevels(keys, n, bucket, i, lo, hi);
-    if (i < n) numS[i] = hi >= lo ? uint32_t(hi - lo + 1) : 0u;
-    const int top = int(waveMax(hi >= lo ? hi : -1));
-    if ((threadIdx.x & 63u) == 0 && top >= 0 &&
-        uint32_t(top) > __hip_atomic_load(maxLevel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(maxLevel, uint32_t(top));
+    __shared__ int wtop[4];
+    int            top = -1;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+    {
+        int lo, hi;
+        splitLevels(keys, n, bucket, i, lo, hi);
+        numS[i] = hi >= lo ? uint32_t(hi - lo + 1) : 0u;
+        if (hi >= lo && hi > top) top = hi;
+    }
+    top = waveMax(top);
+    if ((threadIdx.x & 63u) == 0) wtop[threadIdx.x >> 6] = top;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        top = max(max(wtop[0], wtop[1]), max(wtop[2], wtop[3]));
+        if (top >= 0) atomicMax(maxLevel, uint32_t(top));
+    }
 }
 
 __global__ __launch_bounds__(256) void csEmitChildren(const uint64_t* __restrict__ keys, uint32_t n, uint32_t bucket,
@@ -248,7 +258,8 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
         SPHX_HIP(ctx, hipMemsetAsync(sOfs + nn + 1, 0, sizeof(uint32_t), ctx->stream));
         {
             KTimer kt_(ctx, "csCountSplits");
-            csCountSplits<<<gridFor(nn, 256), 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, sOfs + nn + 1);
+            const unsigned grid = std::min<unsigned>(gridFor(nn, 256), unsigned(ctx->numCU) * 8);
+            csCountSplits<<<grid, 256, 0, ctx->stream>>>(keys, nn, bucketSize, sOfs, sOfs + nn + 1);
         }
         SPHX_LAUNCH_CHECK(ctx, "csCountSplits");
         SPHX_TRY(exclusiveScanU32(ctx, sOfs, sOfs, nn));

@@ -438,6 +438,31 @@ struct SortWork
     uint32_t* hist;
 };
 
+// One pass over the 8-bit digit at `shift`: (kSrc, vSrc) -> (kDst, vDst), stable.
+// tileVar != nullptr: the histogram kernel also writes each tile's OR and AND of the keys.
+static int radixPass(sphx_ctx* ctx, sphx_ctx* tctx, const uint64_t* kSrc, const uint32_t* vSrc, uint64_t* kDst,
+                     uint32_t* vDst, uint64_t n, int shift, uint32_t* hist, uint32_t numTiles, uint64_t* tileVar)
+{
+    {
+        KTimer kt_(tctx, "radixUpsweep");
+        radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles, tileVar);
+    }
+    SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
+    radixColumnScan<<<RADIX, 256, 0, ctx->stream>>>(hist, numTiles);
+    SPHX_LAUNCH_CHECK(ctx, "radixColumnScan");
+    {
+        KTimer kt_(tctx, "radixScatter");
+        if (vSrc)
+            radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
+                                                                          numTiles);
+        else
+            radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
+                                                                           hist, numTiles);
+    }
+    SPHX_LAUNCH_CHECK(ctx, "radixScatter");
+    return SPHX_OK;
+}
+
 // own == nullptr: buffers from the context's scratch, and the passes over digits that are
 // constant across all keys are skipped: the first pass's histogram kernel also forms each
 // tile's OR and AND of the keys, one block reduces them to the bits that vary, and the
@@ -492,12 +517,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
         }
         if (((varies >> shift) & 0xFFull) == 0) continue; // digit constant across all keys
         const bool first = shift == 8 * (lowBits / 8) && tileVar;
-        {
-            KTimer kt_(tctx, "radixUpsweep");
-            radixUpsweep<<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, n, shift, hist, numTiles,
-                                                                     first ? tileVar : nullptr);
-        }
-        SPHX_LAUNCH_CHECK(ctx, "radixUpsweep");
+        SPHX_TRY(radixPass(ctx, tctx, kSrc, vSrc, kDst, vDst, n, shift, hist, numTiles, first ? tileVar : nullptr));
         if (first)
         {
             radixVaryingBits<<<1, 256, 0, ctx->stream>>>(tileVar, numTiles, vary);
@@ -505,21 +525,6 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
             if (ctx->hostStage) SPHX_TRY(readBackStart(ctx, vary, sizeof(uint64_t)));
             pending = true;
         }
-        radixColumnScan<<<RADIX, 256, 0, ctx->stream>>>(hist, numTiles);
-        SPHX_LAUNCH_CHECK(ctx, "radixColumnScan");
-        if (vals)
-            {
-                KTimer kt_(tctx, "radixScatter");
-                radixScatter<true><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, vSrc, kDst, vDst, n, shift, hist,
-                                                                          numTiles);
-            }
-        else
-            {
-                KTimer kt_(tctx, "radixScatter");
-                radixScatter<false><<<numTiles, SORT_THREADS, 0, ctx->stream>>>(kSrc, nullptr, kDst, nullptr, n, shift,
-                                                                           hist, numTiles);
-            }
-        SPHX_LAUNCH_CHECK(ctx, "radixScatter");
         std::swap(kSrc, kDst);
         std::swap(vSrc, vDst);
     }
@@ -535,6 +540,172 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
 int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits)
 {
     return radixSortWith(ctx, keys, vals, n, keyBits, nullptr, std::max(0, std::min(lowBits, keyBits)));
+}
+
+// ---------------------------------------------------------------------------
+// Key sort by the top 32 bits and a run fix-up.  Four passes sort by bits 32..63 (into
+// buffers other than the input, which stays intact); keys with equal top halves then
+// form runs that hold their input order.  When no run is longer than kRunMax, each run
+// is sorted by its low 32 bits, stably, on its way back into the caller's arrays: the
+// result is the stable sort by the whole key, as eight passes give it.  Particle keys of
+// a space-filling curve share their top 32 bits (a cell 2^-10.67 of the box wide) only
+// with a few neighbours, so the fix-up is short; when some run is longer (keys packed
+// far below that resolution, small integer keys), the sort runs its eight passes from
+// the intact input, and the next kPrefixRetryCalls sorts of the context do so directly.
+constexpr uint32_t kRunMax           = 64;
+constexpr uint64_t kPrefixSortMinN   = uint64_t(1) << 18;
+constexpr int      kPrefixRetryCalls = 16;
+constexpr uint32_t kFixOwned = 192; // positions written per wave
+constexpr uint32_t kFixWin   = 256; // the wave's window: 4 keys per lane
+
+__device__ inline uint32_t hi32(uint64_t k) { return uint32_t(k >> 32); }
+__device__ inline uint32_t lo32(uint64_t k) { return uint32_t(k); }
+
+// *flag = 1 if a run of equal top-32-bit keys is longer than kRunMax (keys ordered by
+// their top halves: a run that long has equal top halves kRunMax positions apart)
+__global__ __launch_bounds__(256) void runTooLong(const uint64_t* __restrict__ k, uint32_t n,
+                                                  uint32_t* __restrict__ flag)
+{
+    __shared__ uint32_t any[4];
+    bool                f = false;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i + kRunMax < n; i += gridDim.x * 256u)
+        f = f || hi32(k[i]) == hi32(k[i + kRunMax]);
+    const bool w = __ballot(f) != 0;
+    if ((threadIdx.x & 63u) == 0) any[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0 && (any[0] | any[1] | any[2] | any[3])) atomicOr(flag, 1u);
+}
+
+// exchanges (ka, va) and (kb, vb) when both are in one run and ka's low half is larger
+// (strictly: equal keys keep their order)
+__device__ inline bool runSwap(bool same, uint64_t& ka, uint32_t& va, uint64_t& kb, uint32_t& vb)
+{
+    if (!(same && lo32(ka) > lo32(kb))) return false;
+    const uint64_t tk = ka;
+    const uint32_t tv = va;
+    ka = kb, va = vb, kb = tk, vb = tv;
+    return true;
+}
+
+// One wave per kFixOwned positions [t0, t0 + kFixOwned): its window [t0, t0 + kFixWin)
+// holds every run that starts among them whole (runs are at most kRunMax long).  Element
+// e = 4 lane + s of the window sits at t0 + e.  Odd-even transposition rounds inside the
+// runs repeat until a pair of rounds swaps nothing (a run of length L is sorted after L
+// rounds); the wave then writes the runs that start in [t0, t0 + kFixOwned).  The run
+// that covers t0 from an earlier start belongs to the previous wave, whose window
+// reaches t0 + kFixWin - kFixOwned - 1 >= its end.
+__global__ __launch_bounds__(256) void runFix(const uint64_t* __restrict__ kIn, const uint32_t* __restrict__ vIn,
+                                              uint64_t* __restrict__ kOut, uint32_t* __restrict__ vOut, uint32_t n)
+{
+    static_assert(kFixWin == 4 * 64 && kFixWin - kFixOwned >= kRunMax, "a window holds its owned runs whole");
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t t0   = uint64_t(blockIdx.x * 4u + (threadIdx.x >> 6)) * kFixOwned;
+    if (t0 >= n) return;
+    uint64_t k[4];
+    uint32_t v[4];
+    bool     st[4]; // the element starts a run (padding beyond n: one run each)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+    {
+        const uint64_t p = t0 + 4u * lane + uint32_t(s);
+        const bool     in = p < n;
+        k[s]              = in ? kIn[p] : ~uint64_t(0);
+        v[s]              = in ? vIn[p] : 0u;
+        st[s]             = !in;
+    }
+    // the predecessor of element 0 is t0 - 1 (none for t0 = 0), of 4 lane the previous lane's 3
+    const uint64_t before = t0 > 0 ? kIn[t0 - 1] : 0;
+    const uint64_t prev3  = __shfl_up(k[3], 1, 64);
+    st[0] = st[0] || (lane == 0 ? (t0 == 0 || hi32(before) != hi32(k[0])) : hi32(prev3) != hi32(k[0]));
+#pragma unroll
+    for (int s = 1; s < 4; ++s)
+        st[s] = st[s] || hi32(k[s - 1]) != hi32(k[s]);
+    // pairs inside one run: (4l, 4l+1), (4l+2, 4l+3), (4l+1, 4l+2), (4l+3, 4l+4)
+    const bool s01 = !st[1], s23 = !st[3], s12 = !st[2];
+    const bool nx0 = __shfl_down(st[0] ? 1 : 0, 1, 64) != 0;
+    const bool s30 = lane < 63 && !nx0;
+    for (;;)
+    {
+        bool sw = runSwap(s01, k[0], v[0], k[1], v[1]);
+        sw      = runSwap(s23, k[2], v[2], k[3], v[3]) || sw;
+        sw      = runSwap(s12, k[1], v[1], k[2], v[2]) || sw;
+        // across lanes: lane l's element 3 with lane l + 1's element 0 (both decide alike)
+        const uint64_t kn = __shfl_down(k[0], 1, 64);
+        const uint32_t vn = __shfl_down(v[0], 1, 64);
+        const uint64_t kp = __shfl_up(k[3], 1, 64);
+        const uint32_t vp = __shfl_up(v[3], 1, 64);
+        const bool     sp = __shfl_up(s30 ? 1 : 0, 1, 64) != 0 && lane > 0;
+        const bool     down = s30 && lo32(k[3]) > lo32(kn);
+        const bool     up   = sp && lo32(kp) > lo32(k[0]);
+        if (down) k[3] = kn, v[3] = vn;
+        if (up) k[0] = kp, v[0] = vp;
+        sw = sw || down;
+        if (__ballot(sw) == 0) break;
+    }
+    // owned: runs starting at window positions [f, g), f = the first start, g = the first
+    // start at or after kFixOwned
+    uint32_t f = kFixWin, g = kFixWin;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+    {
+        const uint64_t b  = __ballot(st[s]);
+        const uint64_t bg = b & (~uint64_t(0) << (kFixOwned / 4));
+        if (b) f = min(f, 4u * uint32_t(__builtin_ctzll(b)) + uint32_t(s));
+        if (bg) g = min(g, 4u * uint32_t(__builtin_ctzll(bg)) + uint32_t(s));
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+    {
+        const uint32_t e = 4u * lane + uint32_t(s);
+        const uint64_t p = t0 + e;
+        if (e >= f && e < g && p < n) kOut[p] = k[s], vOut[p] = v[s];
+    }
+}
+
+static int prefixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n)
+{
+    const uint32_t numTiles = uint32_t((n + SORT_TILE - 1) / SORT_TILE);
+    uint64_t*      kA       = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
+    uint32_t*      vA       = scratchT<uint32_t>(ctx, S_SORT_VALS, n);
+    uint64_t*      kB       = scratchT<uint64_t>(ctx, S_SORT_KEYS2, n);
+    uint32_t*      vB       = scratchT<uint32_t>(ctx, S_SORT_VALS2, n);
+    uint32_t*      hist     = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * (numTiles + 1));
+    uint32_t*      flag     = scratchT<uint32_t>(ctx, S_SMALL, 4);
+    if (!kA || !vA || !kB || !vB || !hist || !flag) return SPHX_ERR_NOMEM;
+    SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 32, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, kA, vA, kB, vB, n, 40, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, kB, vB, kA, vA, n, 48, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, kA, vA, kB, vB, n, 56, hist, numTiles, nullptr));
+    SPHX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(uint32_t), ctx->stream));
+    runTooLong<<<std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8), 256, 0, ctx->stream>>>(
+        kB, uint32_t(n), flag);
+    SPHX_LAUNCH_CHECK(ctx, "runTooLong");
+    uint32_t tooLong = 0;
+    SPHX_TRY(readBack(ctx, flag, &tooLong, sizeof(tooLong)));
+    if (tooLong)
+    {
+        ctx->sortPrefixSkip = kPrefixRetryCalls;
+        return radixSortWith(ctx, keys, vals, n, 64, nullptr);
+    }
+    const uint64_t waves = (n + kFixOwned - 1) / kFixOwned;
+    {
+        KTimer kt_(ctx, "runFix");
+        runFix<<<unsigned((waves + 3) / 4), 256, 0, ctx->stream>>>(kB, vB, keys, vals, uint32_t(n));
+    }
+    SPHX_LAUNCH_CHECK(ctx, "runFix");
+    return SPHX_OK;
+}
+
+// sphx_sort_by_key: the prefix sort for large arrays (unless a recent one fell back)
+static int sortKeysValues(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n)
+{
+    if (n < kPrefixSortMinN || n >= (uint64_t(1) << 31)) return radixSortPairs(ctx, keys, vals, n, 64);
+    if (ctx->sortPrefixSkip > 0)
+    {
+        --ctx->sortPrefixSkip;
+        return radixSortPairs(ctx, keys, vals, n, 64);
+    }
+    return prefixSortPairs(ctx, keys, vals, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -948,7 +1119,7 @@ int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n
     if (!ctx) return SPHX_ERR_ARG;
     if (n == 0) return SPHX_OK;
     if (!keys || !values) return argError(ctx, "sort_by_key: null pointer");
-    return radixSortPairs(ctx, keys, values, n, 64);
+    return sortKeysValues(ctx, keys, values, n);
 }
 
 int sphx_sort_by_key_noalloc(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n)

@@ -49,6 +49,8 @@ enum Slot : int
     S_HFAC,          // smoothing-length update factors per neighbor count (FP64, then FP32)
     S_SORT_VAR,      // radix sort: per-tile OR and AND of the keys (first pass)
     S_MINDT,         // SPH loops: per-wave time-step minima before their reduction
+    S_SORT_KEYS2,    // particle-key sort: second ping-pong keys (the input stays intact)
+    S_SORT_VALS2,    // particle-key sort: second ping-pong values
     S_NUM_SLOTS
 };
 
@@ -176,6 +178,7 @@ struct sphx_ctx
     bool            nsValid = false;
     int             fp64Exact = 0; // 1: FP64 loops without FMA contraction (bit-exact vs the CPU path)
     int             loopShape[2] = {0, 0}; // per precision (FP64, MIXED): 0 LDS-staged, 1 sequential
+    int             sortPrefixSkip = 0;    // key sorts left to run whole after a prefix sort fell back
     double          nsTests      = 0;      // candidate tests of the last neighbor search
     // per-kernel HIP-event timing (SPHX_OPT_TIMING)
     struct TimedLaunch

@@ -135,6 +135,57 @@ def test_sort_stable_bit_exact(ctx, n, kind):
     assert np.array_equal(u32(v), vals[order])
 
 
+def _run_keys(rng, n, maxlen, lowvals):
+    """Keys in runs of equal top halves (run lengths 1..maxlen), low halves from lowvals values."""
+    lens, tot = [], 0
+    while tot < n:
+        lens.append(int(rng.integers(1, maxlen + 1)))
+        tot += lens[-1]
+    top = np.sort(rng.choice(2 ** 31, len(lens), replace=False)).astype(np.uint64)
+    hi = np.repeat(top, lens)[:n]
+    return (hi << np.uint64(32)) | rng.integers(0, lowvals, n).astype(np.uint64)
+
+
+# Arrays of at least 2^18 keys sort by their top halves in four passes and fix up the runs of
+# equal top halves (at most 64 long); a longer run sends the sort through all eight passes.
+# (run65_then_runs last: its fallback makes the context's next sorts take all eight passes)
+@pytest.mark.parametrize("kind", ["rand", "runs", "runs_rev", "run64", "ones", "run65_then_runs"])
+def test_sort_prefix_and_runs_bit_exact(ctx, kind):
+    from sphexa_amd import hydro
+
+    n = 300_007
+    print("case prefix sort", kind, flush=True)
+    rng = np.random.default_rng(len(kind))
+    calls = []
+    if kind == "rand":
+        calls.append(rng.integers(0, 2 ** 63, n, dtype=np.uint64))
+    elif kind == "runs":  # many equal low halves: the stable order of equal keys matters
+        calls.append(rng.permutation(_run_keys(rng, n, 64, 8)))
+    elif kind == "runs_rev":
+        calls.append(_run_keys(rng, n, 64, 2 ** 32)[::-1].copy())
+    elif kind == "run64":
+        k = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        k[1000:1064] = (k[1000] & ~np.uint64(0xFFFFFFFF)) | rng.integers(0, 50, 64).astype(np.uint64)
+        calls.append(rng.permutation(k))
+    elif kind == "run65_then_runs":
+        k = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        k[1000:1065] = (k[1000] & ~np.uint64(0xFFFFFFFF)) | rng.integers(0, 50, 65).astype(np.uint64)
+        calls += [rng.permutation(k), rng.permutation(_run_keys(rng, n, 64, 8))]
+    else:  # top halves of all ones, and the largest key
+        k = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        k[:40] = (np.uint64(0xFFFFFFFF) << np.uint64(32)) | rng.integers(0, 2 ** 32, 40).astype(np.uint64)
+        k[40:43] = np.uint64(2 ** 64 - 1)
+        calls.append(rng.permutation(k))
+    for keys in calls:
+        vals = rng.permutation(n).astype(np.uint32)
+        order = np.argsort(keys, kind="stable")
+        k = dev(keys.view(np.int64))
+        v = dev(vals.view(np.int32))
+        hydro.sort_by_key(ctx, k, v)
+        assert np.array_equal(u64(k), keys[order])
+        assert np.array_equal(u32(v), vals[order])
+
+
 # ----------------------------------------------------------------------------- octree
 def _keys(kind, n, seed=5):
     rng = np.random.default_rng(seed)

@@ -136,7 +136,9 @@ enum
     SPHX_OPT_FP64_EXACT = 1, /* 1: FP64 particle loops compiled without FMA contraction, so
                                 every operation rounds like the reference CPU loops (default 0) */
     SPHX_OPT_TIMING     = 2, /* 1: record HIP events on the context stream around every hot
-                                kernel launch (read with sphx_timing_summary) */
+                                kernel launch (read with sphx_timing_summary); 2: around
+                                the neighbor-candidate and SPH particle-loop launches
+                                only; 0: none */
     SPHX_OPT_LOOP_SHAPE_FP64  = 3, /* particle-loop kernel shape for FP64 fields:          */
     SPHX_OPT_LOOP_SHAPE_MIXED = 4  /* 0 = neighbor records staged in LDS per 16-target     */
                                    /* sub-group (default), 1 = one lane per target with    */

@@ -99,7 +99,8 @@ int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value)
     }
     if (option == SPHX_OPT_TIMING)
     {
-        ctx->timing = value ? 1 : 0;
+        if (value < 0 || value > 2) return argError(ctx, "ctx_set_option: timing must be 0, 1 or 2");
+        ctx->timing = value;
         return SPHX_OK;
     }
     if (option == SPHX_OPT_LOOP_SHAPE_FP64 || option == SPHX_OPT_LOOP_SHAPE_MIXED)

@@ -263,11 +263,14 @@ struct KTimer
     sphx_ctx*   ctx;
     const char* name;
     hipEvent_t  a = nullptr;
-    KTimer(sphx_ctx* c, const char* n)
+    // recorded when the context times every hot kernel (timing 1), or only the heavy ones
+    // (timing 2: the neighbor candidates and the SPH particle loops) and this is one of them
+    KTimer(sphx_ctx* c, const char* n, bool particleLoop = false)
         : ctx(c)
         , name(n)
     {
-        if (ctx && ctx->timing && (a = timingEvent(ctx))) (void)hipEventRecord(a, ctx->stream);
+        if (ctx && (ctx->timing == 1 || (ctx->timing == 2 && particleLoop)) && (a = timingEvent(ctx)))
+            (void)hipEventRecord(a, ctx->stream);
     }
     void stop()
     {

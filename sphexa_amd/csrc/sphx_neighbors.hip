@@ -839,7 +839,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
         }
         SPHX_LAUNCH_CHECK(ctx, "nsGroupRangesRetry");
         {
-            KTimer kt_(ctx, "nsCandidates");
+            KTimer kt_(ctx, "nsCandidates", true);
             nsCandidates<Th, Sph><<<numLaunch, 64, 0, ctx->stream>>>(G, list, x, y, z, h, iter, ranges, rcnt, db,
                                                                     ngmax, ng0, hfac, nbrs, lists, nc, outList,
                                                                     flags, gtests);

@@ -125,9 +125,11 @@ class Context:
     def set_fp64_exact(self, on: bool):
         self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_FP64_EXACT, int(bool(on))))
 
-    def set_timing(self, on: bool):
-        """Record HIP events around every hot kernel launch (read with timing_summary)."""
-        self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_TIMING, int(bool(on))))
+    def set_timing(self, on):
+        """Record HIP events around every hot kernel launch (True), around the neighbor-candidate
+        and SPH particle-loop launches only ("loops"), or none (False); read with timing_summary."""
+        mode = 2 if on == "loops" else int(bool(on))
+        self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_TIMING, mode))
 
     def timing_summary(self):
         """{kernel: (launches, total ms)} since the last call; synchronises the stream."""

@@ -496,6 +496,30 @@ def test_hydro_step_spatial_groups_equal_fixed(ctx, case):
         assert np.array_equal(out[True][f], out[False][f]), f
 
 
+def test_timing_modes(ctx):
+    """Kernel timing: every hot kernel (True), the heavy launches only ("loops": neighbor
+    candidates and particle loops), or none; an out-of-range mode is an argument error."""
+    from sphexa_amd import _lib, hydro
+    from sphexa_amd._lib import MIXED
+
+    print("case timing modes", flush=True)
+    inp, box = uniform_box(20000, seed=62)
+    step = hydro.HydroStep(ctx, len(inp["x"]), hbox(box), MIXED)
+    inp = {k: (dev(v) if k in ("x", "y", "z") else dev(v).float()) for k, v in inp.items()}
+    seen = {}
+    for mode in (True, "loops", False):
+        ctx.set_timing(mode)
+        ctx.timing_summary()
+        step.compute_forces(inp)
+        seen[mode] = set(ctx.timing_summary())
+    ctx.set_timing(False)
+    heavy = {"nsCandidates", "densityKernel", "iadKernel", "momentumEnergyKernel"}
+    assert seen["loops"] == heavy
+    assert heavy < seen[True] and "radixScatter" in seen[True]
+    assert seen[False] == set()
+    assert ctx.lib.sphx_ctx_set_option(ctx.h, _lib.OPT_TIMING, 3) == _lib.SPHX_ERR_ARG
+
+
 def test_compute_groups_reference_find_splits_known_answer(ctx):
     """The reference's findSplits known answer on the GPU (see tests/test_oracle.py)."""
     from sphexa_amd import hydro

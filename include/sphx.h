@@ -297,6 +297,15 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                  const sphx_box* box);
 
+/* The four calls above in the order HydroProp::computeForces makes them after the
+ * neighbor search (main/src/propagator/std_hydro.hpp:139-153): density, EOS over the
+ * groups' [first, last), IAD, momentum-energy, on one rank (no halo exchange between
+ * them).  Same results as the four calls.  When the groups cover every particle, the
+ * density loop's epilogue writes the IAD loop's neighbor records (x, y, z, m/rho) and,
+ * for MIXED fields, the IAD loop's epilogue the momentum-energy records, in place of
+ * packing launches.  Synchronises the stream once to return minDtCourant. */
+int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box);
+
 /* ---- integration step ------------------------------------------------------ */
 /* The dataset's global time-step scalars (sph/include/sph/particles_data.hpp:93-111,138). */
 typedef struct sphx_timestep

@@ -105,6 +105,7 @@ SIGNATURES = {
     "sphx_compute_eos": (C.c_int, [_V, _U32, _U32, C.POINTER(Particles)]),
     "sphx_compute_iad": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_momentum_energy": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
+    "sphx_compute_hydro_std": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_density_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V, _V, _V,
                                        _V, _V]),
     "sphx_momentum_energy_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V,

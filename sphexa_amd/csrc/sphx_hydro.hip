@@ -18,10 +18,10 @@ namespace sphx
 {
 namespace exact
 {
-int launchDensity(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&);
-int launchIAD(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&);
+int launchDensity(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&, bool);
+int launchIAD(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&, bool, bool);
 int launchMomentumEnergy(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, const DBox&,
-                         unsigned long long*);
+                         unsigned long long*, bool);
 int launchXmass(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*, const DBox&);
 int launchVeDefGradh(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*, const DBox&);
 int launchIadDivvCurlv(sphx_ctx*, const GroupsDev&, const SphLists&, sphx_particles*, sphx_ve_fields*,
@@ -188,14 +188,12 @@ int sphx_compute_density(sphx_ctx* ctx, const sphx_groups* groups, sphx_particle
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     if (G.numGroups == 0) return SPHX_OK;
     DBox db = makeDBox(box);
-    return useExact(ctx, d) ? exact::launchDensity(ctx, G, lists, d, db) : fast::launchDensity(ctx, G, lists, d, db);
+    return useExact(ctx, d) ? exact::launchDensity(ctx, G, lists, d, db, false)
+                            : fast::launchDensity(ctx, G, lists, d, db, false);
 }
 
-int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d)
+static int launchEos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d)
 {
-    if (!ctx) return SPHX_ERR_ARG;
-    if (!d || !d->u || !d->rho || !d->p || !d->c) return argError(ctx, "compute_eos: null field");
-    if (last < first || last > d->n) return argError(ctx, "compute_eos: bad range");
     if (last == first) return SPHX_OK;
     if (d->precision == SPHX_FP64)
         {
@@ -217,6 +215,15 @@ int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particle
     return SPHX_OK;
 }
 
+int sphx_compute_eos(sphx_ctx* ctx, uint32_t first, uint32_t last, sphx_particles* d)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    if (!d || !d->u || !d->rho || !d->p || !d->c) return argError(ctx, "compute_eos: null field");
+    if (last < first || last > d->n) return argError(ctx, "compute_eos: bad range");
+    SPHX_TRY(launchEos(ctx, first, last, d));
+    return SPHX_OK;
+}
+
 int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
 {
     if (!ctx) return SPHX_ERR_ARG;
@@ -229,7 +236,8 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     if (G.numGroups == 0) return SPHX_OK;
     DBox db = makeDBox(box);
-    return useExact(ctx, d) ? exact::launchIAD(ctx, G, lists, d, db) : fast::launchIAD(ctx, G, lists, d, db);
+    return useExact(ctx, d) ? exact::launchIAD(ctx, G, lists, d, db, false, false)
+                            : fast::launchIAD(ctx, G, lists, d, db, false, false);
 }
 
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
@@ -250,8 +258,49 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     DBox db = makeDBox(box);
     initMinDt<<<1, 64, 0, ctx->stream>>>(dt);
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
-    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt)
-                              : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt));
+    SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt, false)
+                              : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt, false));
+    unsigned long long bits = 0;
+    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    d->minDtCourant = v;
+    return SPHX_OK;
+}
+
+int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
+{
+    if (!ctx) return SPHX_ERR_ARG;
+    SPHX_TRY(checkParticles(ctx, d, "compute_hydro_std: null field"));
+    if (!d->u || !d->vx || !d->vy || !d->vz || !d->rho || !d->p || !d->c || !d->c11 || !d->c12 || !d->c13 ||
+        !d->c22 || !d->c23 || !d->c33 || !d->ax || !d->ay || !d->az || !d->du)
+        return argError(ctx, "compute_hydro_std: null field");
+    if (d->precision != SPHX_FP64 && d->precision != SPHX_MIXED)
+        return argError(ctx, "compute_hydro_std: unknown precision");
+    if (!validBox(box)) return argError(ctx, "compute_hydro_std: invalid box");
+    GroupsDev G;
+    SphLists  lists;
+    SPHX_TRY(listsFor(ctx, groups, d, G, lists));
+    d->minDtCourant = INFINITY;
+    if (G.numGroups == 0) return SPHX_OK;
+    auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
+    if (!dt) return SPHX_ERR_NOMEM;
+    // every particle a target: the density loop's epilogue writes the IAD records and, in
+    // mixed precision, the IAD loop's epilogue the momentum-energy records (the 144-byte
+    // FP64 records cost the FP64 IAD epilogue more than their packing launch: measured)
+    const bool handOff = G.first == 0 && G.last == d->n;
+    const bool meOff   = handOff && d->precision == SPHX_MIXED;
+    const bool ex      = useExact(ctx, d);
+    DBox       db      = makeDBox(box);
+    SPHX_TRY(ex ? exact::launchDensity(ctx, G, lists, d, db, handOff)
+                : fast::launchDensity(ctx, G, lists, d, db, handOff));
+    SPHX_TRY(launchEos(ctx, G.first, G.last, d));
+    SPHX_TRY(ex ? exact::launchIAD(ctx, G, lists, d, db, handOff, meOff)
+                : fast::launchIAD(ctx, G, lists, d, db, handOff, meOff));
+    initMinDt<<<1, 64, 0, ctx->stream>>>(dt);
+    SPHX_LAUNCH_CHECK(ctx, "initMinDt");
+    SPHX_TRY(ex ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt, meOff)
+                : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt, meOff));
     unsigned long long bits = 0;
     SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
     double v;

@@ -545,6 +545,16 @@ def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups
     return d.minDtCourant
 
 
+def compute_hydro_std(ctx: Context, d: ParticlesData, box: Box, groups=None):
+    """Density, EOS, IAD and momentum-energy in one call, the order of HydroProp::computeForces
+    (main/src/propagator/std_hydro.hpp:139-153); same results as the four calls, with each
+    loop writing the next loop's neighbor records when the groups cover every particle."""
+    s = d.struct()
+    ctx.check(ctx.lib.sphx_compute_hydro_std(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box)))
+    d.minDtCourant = s.minDtCourant
+    return d.minDtCourant
+
+
 # ------------------------------------------------------------------------------------------- integration
 @dataclass
 class TimestepState:
@@ -830,14 +840,17 @@ class HydroStep:
         mark(8)
         find_neighbors_sfc(ctx, d, box, o, g)
         mark(9)
-        compute_density(ctx, d, box, g)
-        mark(10)
-        compute_eos_hydro_std(ctx, d)
-        mark(11)
-        compute_iad(ctx, d, box, g)
-        mark(12)
-        compute_momentum_energy_std(ctx, d, box, g)
-        mark(13)
+        if timed:  # the four calls, for the per-stage times
+            compute_density(ctx, d, box, g)
+            mark(10)
+            compute_eos_hydro_std(ctx, d)
+            mark(11)
+            compute_iad(ctx, d, box, g)
+            mark(12)
+            compute_momentum_energy_std(ctx, d, box, g)
+            mark(13)
+        else:
+            compute_hydro_std(ctx, d, box, g)
         self.events = ev
         return d
 

@@ -496,6 +496,33 @@ def test_hydro_step_spatial_groups_equal_fixed(ctx, case):
         assert np.array_equal(out[True][f], out[False][f]), f
 
 
+@pytest.mark.parametrize("mode", ["mixed", "fp64", "fp64_exact"])
+def test_hydro_std_one_call_equals_four_calls(ctx, mode):
+    """sphx_compute_hydro_std (each loop writing the next loop's neighbor records) gives the
+    bits of the four calls density, EOS, IAD, momentum-energy (each packing its records)."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import FP64, MIXED
+
+    print("case hydro_std one call", mode, flush=True)
+    inp, box = uniform_box(30000, seed=63)
+    prec = MIXED if mode == "mixed" else FP64
+    tin = {k: (dev(v) if (k in ("x", "y", "z") or prec == FP64) else dev(v).float()) for k, v in inp.items()}
+    ctx.set_fp64_exact(mode == "fp64_exact")
+    try:
+        step = hydro.HydroStep(ctx, len(inp["x"]), hbox(box), prec)
+        out = {}
+        for timed in (True, False):  # timed: the four calls (per-stage events); else the one call
+            d = step.compute_forces(tin, timed=timed)
+            torch.cuda.synchronize()
+            out[timed] = {f: getattr(d, f).cpu().numpy().copy() for f in FIELDS + ("p", "c")}
+            out[timed]["dt"] = d.minDtCourant
+    finally:
+        ctx.set_fp64_exact(False)
+    for f in FIELDS + ("p", "c"):
+        assert np.array_equal(out[True][f], out[False][f]), f
+    assert out[True]["dt"] == out[False]["dt"]
+
+
 def test_timing_modes(ctx):
     """Kernel timing: every hot kernel (True), the heavy launches only ("loops": neighbor
     candidates and particle loops), or none; an out-of-range mode is an argument error."""

@@ -543,15 +543,16 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
 }
 
 // ---------------------------------------------------------------------------
-// Key sort by the top 32 bits and a run fix-up.  Four passes sort by bits 32..63 (into
-// buffers other than the input, which stays intact); keys with equal top halves then
-// form runs that hold their input order.  When no run is longer than kRunMax, each run
-// is sorted by its low 32 bits, stably, on its way back into the caller's arrays: the
-// result is the stable sort by the whole key, as eight passes give it.  Particle keys of
-// a space-filling curve share their top 32 bits (a cell 2^-10.67 of the box wide) only
-// with a few neighbours, so the fix-up is short; when some run is longer (keys packed
-// far below that resolution, small integer keys), the sort runs its eight passes from
-// the intact input, and the next kPrefixRetryCalls sorts of the context do so directly.
+// Key sort by the top 32 bits and a run fix-up.  Four passes sort by bits 32..63, ending
+// in the caller's arrays; keys with equal top halves then form runs that hold their input
+// order.  When no run is longer than kRunMax, each run is sorted by its low 32 bits,
+// stably, in place: the result is the stable sort by the whole key, as eight passes give
+// it.  Particle keys of a space-filling curve share their top 32 bits (a cell 2^-10.67 of
+// the box wide) only with a few neighbours, so the fix-up is short; when some run is
+// longer (keys packed far below that resolution, small integer keys), the eight passes
+// run on the arrays as the four left them, which gives the same result (a stable sort of
+// a permutation whose equal keys keep their input order), and the next
+// kPrefixRetryCalls sorts of the context run them directly.
 constexpr uint32_t kRunMax           = 64;
 constexpr uint64_t kPrefixSortMinN   = uint64_t(1) << 18;
 constexpr int      kPrefixRetryCalls = 16;
@@ -587,34 +588,35 @@ __device__ inline bool runSwap(bool same, uint64_t& ka, uint32_t& va, uint64_t& 
     return true;
 }
 
-// One wave per kFixOwned positions [t0, t0 + kFixOwned): its window [t0, t0 + kFixWin)
-// holds every run that starts among them whole (runs are at most kRunMax long).  Element
-// e = 4 lane + s of the window sits at t0 + e.  Odd-even transposition rounds inside the
-// runs repeat until a pair of rounds swaps nothing (a run of length L is sorted after L
-// rounds); the wave then writes the runs that start in [t0, t0 + kFixOwned).  The run
-// that covers t0 from an earlier start belongs to the previous wave, whose window
-// reaches t0 + kFixWin - kFixOwned - 1 >= its end.
-__global__ __launch_bounds__(256) void runFix(const uint64_t* __restrict__ kIn, const uint32_t* __restrict__ vIn,
-                                              uint64_t* __restrict__ kOut, uint32_t* __restrict__ vOut, uint32_t n)
+// In place.  One wave per kFixOwned positions [t0, t0 + kFixOwned): its window
+// [t0, t0 + kFixWin) holds every run that starts among them whole (runs are at most
+// kRunMax long).  Element e = 4 lane + s of the window sits at t0 + e.  Odd-even
+// transposition rounds inside the runs repeat until a pair of rounds swaps nothing (a
+// run of length L is sorted after L rounds); the wave then writes the keys that moved in
+// the runs that start in [t0, t0 + kFixOwned).  The run that covers t0 from an earlier
+// start belongs to the previous wave, whose window reaches t0 + kFixWin - kFixOwned - 1
+// >= its end.  A wave writes only its own runs' positions; of its neighbours' positions
+// it uses only the top halves, which the fix-up leaves unchanged at every position.
+__global__ __launch_bounds__(256) void runFix(uint64_t* keys, uint32_t* vals, uint32_t n)
 {
     static_assert(kFixWin == 4 * 64 && kFixWin - kFixOwned >= kRunMax, "a window holds its owned runs whole");
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t t0   = uint64_t(blockIdx.x * 4u + (threadIdx.x >> 6)) * kFixOwned;
     if (t0 >= n) return;
-    uint64_t k[4];
-    uint32_t v[4];
+    uint64_t k[4], k0[4];
+    uint32_t v[4], v0[4];
     bool     st[4]; // the element starts a run (padding beyond n: one run each)
 #pragma unroll
     for (int s = 0; s < 4; ++s)
     {
         const uint64_t p = t0 + 4u * lane + uint32_t(s);
         const bool     in = p < n;
-        k[s]              = in ? kIn[p] : ~uint64_t(0);
-        v[s]              = in ? vIn[p] : 0u;
+        k[s] = k0[s]      = in ? keys[p] : ~uint64_t(0);
+        v[s] = v0[s]      = in ? vals[p] : 0u;
         st[s]             = !in;
     }
     // the predecessor of element 0 is t0 - 1 (none for t0 = 0), of 4 lane the previous lane's 3
-    const uint64_t before = t0 > 0 ? kIn[t0 - 1] : 0;
+    const uint64_t before = t0 > 0 ? keys[t0 - 1] : 0;
     const uint64_t prev3  = __shfl_up(k[3], 1, 64);
     st[0] = st[0] || (lane == 0 ? (t0 == 0 || hi32(before) != hi32(k[0])) : hi32(prev3) != hi32(k[0]));
 #pragma unroll
@@ -658,7 +660,7 @@ __global__ __launch_bounds__(256) void runFix(const uint64_t* __restrict__ kIn, 
     {
         const uint32_t e = 4u * lane + uint32_t(s);
         const uint64_t p = t0 + e;
-        if (e >= f && e < g && p < n) kOut[p] = k[s], vOut[p] = v[s];
+        if (e >= f && e < g && p < n && (k[s] != k0[s] || v[s] != v0[s])) keys[p] = k[s], vals[p] = v[s];
     }
 }
 
@@ -667,18 +669,16 @@ static int prefixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64
     const uint32_t numTiles = uint32_t((n + SORT_TILE - 1) / SORT_TILE);
     uint64_t*      kA       = scratchT<uint64_t>(ctx, S_SORT_KEYS, n);
     uint32_t*      vA       = scratchT<uint32_t>(ctx, S_SORT_VALS, n);
-    uint64_t*      kB       = scratchT<uint64_t>(ctx, S_SORT_KEYS2, n);
-    uint32_t*      vB       = scratchT<uint32_t>(ctx, S_SORT_VALS2, n);
     uint32_t*      hist     = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * (numTiles + 1));
     uint32_t*      flag     = scratchT<uint32_t>(ctx, S_SMALL, 4);
-    if (!kA || !vA || !kB || !vB || !hist || !flag) return SPHX_ERR_NOMEM;
+    if (!kA || !vA || !hist || !flag) return SPHX_ERR_NOMEM;
     SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 32, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, kA, vA, kB, vB, n, 40, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, kB, vB, kA, vA, n, 48, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, kA, vA, kB, vB, n, 56, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 40, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 48, hist, numTiles, nullptr));
+    SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 56, hist, numTiles, nullptr));
     SPHX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(uint32_t), ctx->stream));
     runTooLong<<<std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8), 256, 0, ctx->stream>>>(
-        kB, uint32_t(n), flag);
+        keys, uint32_t(n), flag);
     SPHX_LAUNCH_CHECK(ctx, "runTooLong");
     uint32_t tooLong = 0;
     SPHX_TRY(readBack(ctx, flag, &tooLong, sizeof(tooLong)));
@@ -690,7 +690,7 @@ static int prefixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64
     const uint64_t waves = (n + kFixOwned - 1) / kFixOwned;
     {
         KTimer kt_(ctx, "runFix");
-        runFix<<<unsigned((waves + 3) / 4), 256, 0, ctx->stream>>>(kB, vB, keys, vals, uint32_t(n));
+        runFix<<<unsigned((waves + 3) / 4), 256, 0, ctx->stream>>>(keys, vals, uint32_t(n));
     }
     SPHX_LAUNCH_CHECK(ctx, "runFix");
     return SPHX_OK;

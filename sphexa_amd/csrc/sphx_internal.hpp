@@ -49,8 +49,6 @@ enum Slot : int
     S_HFAC,          // smoothing-length update factors per neighbor count (FP64, then FP32)
     S_SORT_VAR,      // radix sort: per-tile OR and AND of the keys (first pass)
     S_MINDT,         // SPH loops: per-wave time-step minima before their reduction
-    S_SORT_KEYS2,    // particle-key sort: second ping-pong keys (the input stays intact)
-    S_SORT_VALS2,    // particle-key sort: second ping-pong values
     S_HY_REC_IAD,    // fused std-hydro step: IAD records written by the density loop
     S_HY_REC_ME,     // fused std-hydro step: momentum-energy records written by the IAD loop
     S_NUM_SLOTS

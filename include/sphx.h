@@ -187,8 +187,12 @@ int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const
 
 /* ---- sort / permute ------------------------------------------------------ */
 /* cstone::sortByKey<true> (domain/include/cstone/primitives/primitives_acc.hpp:115-121,
- * primitives_gpu.cu:308-339): stable ascending LSD radix sort of (key, value)
- * pairs, in place.  Scratch comes from the context. */
+ * primitives_gpu.cu:308-339): stable ascending radix sort of (key, value) pairs, in
+ * place.  From 2^18 pairs on, four passes sort by the top 32 key bits and runs of equal
+ * top halves up to 64 long are then sorted by the low halves (a longer run makes the
+ * call run all eight passes); the result is the same.  Synchronises the stream once
+ * (the digit check of the eight-pass sort, or the run-length check).  Scratch comes
+ * from the context. */
 int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code

@@ -523,6 +523,38 @@ def test_hydro_std_one_call_equals_four_calls(ctx, mode):
     assert out[True]["dt"] == out[False]["dt"]
 
 
+def test_hydro_std_one_call_sub_range(ctx):
+    """Groups over a sub-range [first, last) (the particles outside are neighbors only): the
+    one call packs the records as the four calls do and gives their bits on [first, last)."""
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import MIXED
+
+    print("case hydro_std one call sub-range", flush=True)
+    inp, box = uniform_box(30000, seed=64)
+    n = len(inp["x"])
+    tin = {k: (dev(v) if k in ("x", "y", "z") else dev(v).float()) for k, v in inp.items()}
+    step = hydro.HydroStep(ctx, n, hbox(box), MIXED)
+    d = step.compute_forces(tin)  # sorted fields, tree, and every field set once
+    first, last = 1000, n - 700
+    g = hydro.compute_groups(ctx, d.x, d.y, d.z, d.h, step.octree, step.box, first=first, last=last)
+    hydro.find_neighbors_sfc(ctx, d, step.box, step.octree, g)
+    out = {}
+    for one in (False, True):
+        if one:
+            hydro.compute_hydro_std(ctx, d, step.box, g)
+        else:
+            hydro.compute_density(ctx, d, step.box, g)
+            hydro.compute_eos_hydro_std(ctx, d, first, last)
+            hydro.compute_iad(ctx, d, step.box, g)
+            hydro.compute_momentum_energy_std(ctx, d, step.box, g)
+        torch.cuda.synchronize()
+        out[one] = {f: getattr(d, f).cpu().numpy()[first:last].copy() for f in FIELDS + ("p", "c")}
+        out[one]["dt"] = d.minDtCourant
+    for f in FIELDS + ("p", "c"):
+        assert np.array_equal(out[True][f], out[False][f]), f
+    assert out[True]["dt"] == out[False]["dt"]
+
+
 def test_timing_modes(ctx):
     """Kernel timing: every hot kernel (True), the heavy launches only ("loops": neighbor
     candidates and particle loops), or none; an out-of-range mode is an argument error."""

@@ -543,61 +543,67 @@ int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, in
 }
 
 // ---------------------------------------------------------------------------
-// Key sort by the top 32 bits and a run fix-up.  Four passes sort by bits 32..63, ending
-// in the caller's arrays; keys with equal top halves then form runs that hold their input
-// order.  When no run is longer than kRunMax, each run is sorted by its low 32 bits,
-// stably, in place: the result is the stable sort by the whole key, as eight passes give
-// it.  Particle keys of a space-filling curve share their top 32 bits (a cell 2^-10.67 of
-// the box wide) only with a few neighbours, so the fix-up is short; when some run is
-// longer (keys packed far below that resolution, small integer keys), the eight passes
-// run on the arrays as the four left them, which gives the same result (a stable sort of
-// a permutation whose equal keys keep their input order), and the next
-// kPrefixRetryCalls sorts of the context run them directly.
+// Key sort by a top prefix and a run fix-up.  Four passes sort by bits 32..63 (or, when the
+// context's previous prefix sort found every run of equal top 24 bits short, three passes
+// by bits 40..63 through two pairs of work arrays), ending in the caller's arrays; keys
+// with an equal prefix then form runs that hold their input order.  When no run is
+// longer than kRunMax, each run is sorted by the bits below the prefix, stably: the
+// result is the stable sort by the whole key, as eight passes give it.  Particle keys of
+// a space-filling curve share their top 32 bits (a cell 2^-10.67 of the box wide), or top
+// 24 bits (2^-7.67 of the box), only with a few neighbours, so the fix-up is short; when
+// some run is longer (keys packed far below that resolution, small integer keys), the
+// eight passes run on the arrays as the prefix passes left them, which gives the same
+// result (a stable sort of a permutation whose equal keys keep their input order): after
+// four prefix passes the next kPrefixRetryCalls sorts of the context run them directly,
+// after three the next sort takes four.
 constexpr uint32_t kRunMax           = 64;
 constexpr uint64_t kPrefixSortMinN   = uint64_t(1) << 18;
 constexpr int      kPrefixRetryCalls = 16;
 constexpr uint32_t kFixOwned = 192; // positions written per wave
 constexpr uint32_t kFixWin   = 256; // the wave's window: 4 keys per lane
 
-__device__ inline uint32_t hi32(uint64_t k) { return uint32_t(k >> 32); }
-__device__ inline uint32_t lo32(uint64_t k) { return uint32_t(k); }
 
-// *flag = 1 if a run of equal top-32-bit keys is longer than kRunMax (keys ordered by
-// their top halves: a run that long has equal top halves kRunMax positions apart)
-__global__ __launch_bounds__(256) void runTooLong(const uint64_t* __restrict__ k, uint32_t n,
+// flag bit 0: a run of keys with equal bits sh..63 is longer than kRunMax; bit 1: the same
+// for bits 40..63 (keys ordered by bits sh..63, sh <= 40: a run that long has equal
+// prefixes kRunMax positions apart)
+__global__ __launch_bounds__(256) void runTooLong(const uint64_t* __restrict__ k, uint32_t n, uint32_t sh,
                                                   uint32_t* __restrict__ flag)
 {
     __shared__ uint32_t any[4];
-    bool                f = false;
+    uint32_t            f = 0;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i + kRunMax < n; i += gridDim.x * 256u)
-        f = f || hi32(k[i]) == hi32(k[i + kRunMax]);
-    const bool w = __ballot(f) != 0;
+    {
+        const uint64_t a = k[i], b = k[i + kRunMax];
+        f |= ((a >> sh) == (b >> sh) ? 1u : 0u) | ((a >> 40) == (b >> 40) ? 2u : 0u);
+    }
+    const uint32_t w = (__ballot(f & 1u) != 0 ? 1u : 0u) | (__ballot(f & 2u) != 0 ? 2u : 0u);
     if ((threadIdx.x & 63u) == 0) any[threadIdx.x >> 6] = w;
     __syncthreads();
-    if (threadIdx.x == 0 && (any[0] | any[1] | any[2] | any[3])) atomicOr(flag, 1u);
+    if (threadIdx.x == 0 && (any[0] | any[1] | any[2] | any[3])) atomicOr(flag, any[0] | any[1] | any[2] | any[3]);
 }
 
-// exchanges (ka, va) and (kb, vb) when both are in one run and ka's low half is larger
-// (strictly: equal keys keep their order)
+// exchanges (ka, va) and (kb, vb) when both are in one run and ka is larger (within a run
+// the prefixes are equal, so the bits below decide; strictly: equal keys keep their order)
 __device__ inline bool runSwap(bool same, uint64_t& ka, uint32_t& va, uint64_t& kb, uint32_t& vb)
 {
-    if (!(same && lo32(ka) > lo32(kb))) return false;
+    if (!(same && ka > kb)) return false;
     const uint64_t tk = ka;
     const uint32_t tv = va;
     ka = kb, va = vb, kb = tk, vb = tv;
     return true;
 }
 
-// In place.  One wave per kFixOwned positions [t0, t0 + kFixOwned): its window
-// [t0, t0 + kFixWin) holds every run that starts among them whole (runs are at most
-// kRunMax long).  Element e = 4 lane + s of the window sits at t0 + e.  Odd-even
-// transposition rounds inside the runs repeat until a pair of rounds swaps nothing (a
-// run of length L is sorted after L rounds); the wave then writes the keys that moved in
-// the runs that start in [t0, t0 + kFixOwned).  The run that covers t0 from an earlier
-// start belongs to the previous wave, whose window reaches t0 + kFixWin - kFixOwned - 1
-// >= its end.  A wave writes only its own runs' positions; of its neighbours' positions
-// it uses only the top halves, which the fix-up leaves unchanged at every position.
-__global__ __launch_bounds__(256) void runFix(uint64_t* keys, uint32_t* vals, uint32_t n)
+// Runs are keys with equal bits sh..63.  In place.  One wave per kFixOwned positions
+// [t0, t0 + kFixOwned): its window [t0, t0 + kFixWin) holds every run that starts among
+// them whole (runs are at most kRunMax long).  Element e = 4 lane + s of the window sits
+// at t0 + e.  Odd-even transposition rounds inside the runs repeat until a pair of rounds
+// swaps nothing (a run of length L is sorted after L rounds); the wave then writes the
+// keys that moved in the runs that start in [t0, t0 + kFixOwned).  The run that covers t0
+// from an earlier start belongs to the previous wave, whose window reaches
+// t0 + kFixWin - kFixOwned - 1 >= its end.  A wave writes only its own runs' positions;
+// of its neighbours' positions it uses only the prefixes, which the fix-up leaves
+// unchanged at every position.
+__global__ __launch_bounds__(256) void runFix(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t sh)
 {
     static_assert(kFixWin == 4 * 64 && kFixWin - kFixOwned >= kRunMax, "a window holds its owned runs whole");
     const uint32_t lane = threadIdx.x & 63u;
@@ -618,10 +624,10 @@ __global__ __launch_bounds__(256) void runFix(uint64_t* keys, uint32_t* vals, ui
     // the predecessor of element 0 is t0 - 1 (none for t0 = 0), of 4 lane the previous lane's 3
     const uint64_t before = t0 > 0 ? keys[t0 - 1] : 0;
     const uint64_t prev3  = __shfl_up(k[3], 1, 64);
-    st[0] = st[0] || (lane == 0 ? (t0 == 0 || hi32(before) != hi32(k[0])) : hi32(prev3) != hi32(k[0]));
+    st[0] = st[0] || (lane == 0 ? (t0 == 0 || (before >> sh) != (k[0] >> sh)) : (prev3 >> sh) != (k[0] >> sh));
 #pragma unroll
     for (int s = 1; s < 4; ++s)
-        st[s] = st[s] || hi32(k[s - 1]) != hi32(k[s]);
+        st[s] = st[s] || (k[s - 1] >> sh) != (k[s] >> sh);
     // pairs inside one run: (4l, 4l+1), (4l+2, 4l+3), (4l+1, 4l+2), (4l+3, 4l+4)
     const bool s01 = !st[1], s23 = !st[3], s12 = !st[2];
     const bool nx0 = __shfl_down(st[0] ? 1 : 0, 1, 64) != 0;
@@ -637,8 +643,8 @@ __global__ __launch_bounds__(256) void runFix(uint64_t* keys, uint32_t* vals, ui
         const uint64_t kp = __shfl_up(k[3], 1, 64);
         const uint32_t vp = __shfl_up(v[3], 1, 64);
         const bool     sp = __shfl_up(s30 ? 1 : 0, 1, 64) != 0 && lane > 0;
-        const bool     down = s30 && lo32(k[3]) > lo32(kn);
-        const bool     up   = sp && lo32(kp) > lo32(k[0]);
+        const bool     down = s30 && k[3] > kn;
+        const bool     up   = sp && kp > k[0];
         if (down) k[3] = kn, v[3] = vn;
         if (up) k[0] = kp, v[0] = vp;
         sw = sw || down;
@@ -672,25 +678,41 @@ static int prefixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64
     uint32_t*      hist     = scratchT<uint32_t>(ctx, S_SORT_HIST, uint64_t(RADIX) * (numTiles + 1));
     uint32_t*      flag     = scratchT<uint32_t>(ctx, S_SMALL, 4);
     if (!kA || !vA || !hist || !flag) return SPHX_ERR_NOMEM;
-    SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 32, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 40, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 48, hist, numTiles, nullptr));
-    SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 56, hist, numTiles, nullptr));
+    const bool     p24 = ctx->sortPrefix24;
+    const uint32_t sh  = p24 ? 40u : 32u;
+    if (p24)
+    {
+        uint64_t* kB = scratchT<uint64_t>(ctx, S_SORT_KEYS2, n);
+        uint32_t* vB = scratchT<uint32_t>(ctx, S_SORT_VALS2, n);
+        if (!kB || !vB) return SPHX_ERR_NOMEM;
+        SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 40, hist, numTiles, nullptr));
+        SPHX_TRY(radixPass(ctx, ctx, kA, vA, kB, vB, n, 48, hist, numTiles, nullptr));
+        SPHX_TRY(radixPass(ctx, ctx, kB, vB, keys, vals, n, 56, hist, numTiles, nullptr));
+    }
+    else
+    {
+        SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 32, hist, numTiles, nullptr));
+        SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 40, hist, numTiles, nullptr));
+        SPHX_TRY(radixPass(ctx, ctx, keys, vals, kA, vA, n, 48, hist, numTiles, nullptr));
+        SPHX_TRY(radixPass(ctx, ctx, kA, vA, keys, vals, n, 56, hist, numTiles, nullptr));
+    }
     SPHX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(uint32_t), ctx->stream));
     runTooLong<<<std::min<unsigned>(gridFor(n, 256), unsigned(ctx->numCU) * 8), 256, 0, ctx->stream>>>(
-        keys, uint32_t(n), flag);
+        keys, uint32_t(n), sh, flag);
     SPHX_LAUNCH_CHECK(ctx, "runTooLong");
     uint32_t tooLong = 0;
     SPHX_TRY(readBack(ctx, flag, &tooLong, sizeof(tooLong)));
-    if (tooLong)
+    // the next prefix sort takes three passes when every run of equal top 24 bits is short
+    ctx->sortPrefix24 = (tooLong & 2u) == 0;
+    if (tooLong & 1u)
     {
-        ctx->sortPrefixSkip = kPrefixRetryCalls;
+        if (!p24) ctx->sortPrefixSkip = kPrefixRetryCalls;
         return radixSortWith(ctx, keys, vals, n, 64, nullptr);
     }
     const uint64_t waves = (n + kFixOwned - 1) / kFixOwned;
     {
         KTimer kt_(ctx, "runFix");
-        runFix<<<unsigned((waves + 3) / 4), 256, 0, ctx->stream>>>(keys, vals, uint32_t(n));
+        runFix<<<unsigned((waves + 3) / 4), 256, 0, ctx->stream>>>(keys, vals, uint32_t(n), sh);
     }
     SPHX_LAUNCH_CHECK(ctx, "runFix");
     return SPHX_OK;

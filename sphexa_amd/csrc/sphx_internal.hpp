@@ -51,6 +51,8 @@ enum Slot : int
     S_MINDT,         // SPH loops: per-wave time-step minima before their reduction
     S_HY_REC_IAD,    // fused std-hydro step: IAD records written by the density loop
     S_HY_REC_ME,     // fused std-hydro step: momentum-energy records written by the IAD loop
+    S_SORT_KEYS2,    // three-pass prefix sort: second work keys
+    S_SORT_VALS2,    // three-pass prefix sort: second work values
     S_NUM_SLOTS
 };
 
@@ -179,6 +181,7 @@ struct sphx_ctx
     int             fp64Exact = 0; // 1: FP64 loops without FMA contraction (bit-exact vs the CPU path)
     int             loopShape[2] = {0, 0}; // per precision (FP64, MIXED): 0 LDS-staged, 1 sequential
     int             sortPrefixSkip = 0;    // key sorts left to run whole after a prefix sort fell back
+    bool            sortPrefix24   = false; // the last prefix sort found every 24-bit prefix run short
     double          nsTests      = 0;      // candidate tests of the last neighbor search
     // per-kernel HIP-event timing (SPHX_OPT_TIMING)
     struct TimedLaunch

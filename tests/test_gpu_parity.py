@@ -186,6 +186,70 @@ def test_sort_prefix_and_runs_bit_exact(ctx, kind):
         assert np.array_equal(u32(v), vals[order])
 
 
+def _prefix_runs(rng, n, maxlen, lowvals, prefix_bits=24):
+    """Keys in runs of equal top prefix_bits bits (run lengths 1..maxlen), the bits below
+    from lowvals values."""
+    lens, tot = [], 0
+    while tot < n:
+        lens.append(int(rng.integers(1, maxlen + 1)))
+        tot += lens[-1]
+    top = np.sort(rng.choice(2 ** (prefix_bits - 1), len(lens), replace=False)).astype(np.uint64)
+    return (np.repeat(top, lens)[:n] << np.uint64(64 - prefix_bits)) | rng.integers(0, lowvals, n).astype(np.uint64)
+
+
+# After a prefix sort whose runs of equal top 24 bits were all short, the context's next sort
+# takes three passes (bits 40..63) and fixes up the runs from the work arrays into the
+# caller's; a longer 24-bit run there sends it through the eight passes (after a copy back)
+# and the sort after that takes four passes again.  The scatter launches show the path.
+def test_sort_three_pass_prefix_bit_exact():
+    from sphexa_amd import hydro
+
+    n = 300_007
+    rng = np.random.default_rng(24)
+    c = hydro.Context(0)
+    try:
+        k65 = rng.integers(0, 2 ** 63, n, dtype=np.uint64)  # one 24-bit run of 65, 32-bit runs short
+        k65[1000:1065] = (k65[1000] & ~np.uint64(2 ** 40 - 1)) | (
+            rng.choice(2 ** 32, 65, replace=False).astype(np.uint64) << np.uint64(8))
+        k64 = rng.integers(0, 2 ** 63, n, dtype=np.uint64)  # one 24-bit run of exactly 64
+        k64[5:69] = (k64[5] & ~np.uint64(2 ** 40 - 1)) | rng.integers(0, 40, 64).astype(np.uint64)
+        ones = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        ones[:40] = (np.uint64(0xFFFFFF) << np.uint64(40)) | rng.integers(0, 2 ** 40, 40).astype(np.uint64)
+        ones[40:43] = np.uint64(2 ** 64 - 1)
+        calls = [  # (keys, prefix passes expected)
+            ("rand", rng.integers(0, 2 ** 63, n, dtype=np.uint64), 4),
+            ("runs24", rng.permutation(_prefix_runs(rng, n, 64, 8)), 3),
+            ("runs24_rev", _prefix_runs(rng, n, 64, 2 ** 40)[::-1].copy(), 3),
+            ("run24_64", rng.permutation(k64), 3),
+            ("ones24", rng.permutation(ones), 3),
+            ("run24_65", rng.permutation(k65), 3),
+            ("runs24_after_fallback", rng.permutation(_prefix_runs(rng, n, 64, 8)), 4),
+            ("rand_again", rng.integers(0, 2 ** 63, n, dtype=np.uint64), 3),
+        ]
+        for name, keys, passes in calls:
+            print("case three-pass prefix sort", name, flush=True)
+            vals = rng.permutation(n).astype(np.uint32)
+            order = np.argsort(keys, kind="stable")
+            k = dev(keys.view(np.int64))
+            v = dev(vals.view(np.int32))
+            c.set_timing(True)
+            c.timing_summary()
+            hydro.sort_by_key(c, k, v)
+            t = c.timing_summary()
+            c.set_timing(False)
+            assert np.array_equal(u64(k), keys[order]), name
+            assert np.array_equal(u32(v), vals[order]), name
+            scatters = t["radixScatter"][0]
+            if name == "run24_65":
+                assert scatters > 3, (name, scatters)  # the prefix passes, then the full sort
+                assert "runFix" not in t, name
+            else:
+                assert scatters == passes, (name, scatters)
+                assert t["runFix"][0] == 1, name
+    finally:
+        c.close()
+
+
 # ----------------------------------------------------------------------------- octree
 def _keys(kind, n, seed=5):
     rng = np.random.default_rng(seed)

@@ -190,9 +190,11 @@ int sphx_compute_sfc_keys(sphx_ctx* ctx, const double* x, const double* y, const
  * primitives_gpu.cu:308-339): stable ascending radix sort of (key, value) pairs, in
  * place.  From 2^18 pairs on, four passes sort by the top 32 key bits and runs of equal
  * top halves up to 64 long are then sorted by the low halves (a longer run makes the
- * call run all eight passes); the result is the same.  Synchronises the stream once
- * (the digit check of the eight-pass sort, or the run-length check).  Scratch comes
- * from the context. */
+ * call run all eight passes); when the context's previous such sort found every run of
+ * equal top 24 bits at most 64 long, three passes sort by the top 24 bits and those runs
+ * are fixed up instead.  The result is the same.  Synchronises the stream once (the
+ * digit check of the eight-pass sort, or the run-length check).  Scratch comes from the
+ * context. */
 int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code

@@ -140,11 +140,23 @@ enum
                                 the neighbor-candidate and SPH particle-loop launches
                                 only; 0: none */
     SPHX_OPT_LOOP_SHAPE_FP64  = 3, /* particle-loop kernel shape for FP64 fields:          */
-    SPHX_OPT_LOOP_SHAPE_MIXED = 4  /* 0 = neighbor records staged in LDS per 16-target     */
+    SPHX_OPT_LOOP_SHAPE_MIXED = 4, /* 0 = neighbor records staged in LDS per 16-target     */
                                    /* sub-group (default), 1 = one lane per target with    */
                                    /* neighbor records gathered from global memory         */
+    SPHX_OPT_DEFER_MIN_DT = 5      /* 1: the momentum-energy entries do not wait for the
+                                      Courant time step: d->minDtCourant is set to NaN and
+                                      sphx_min_dt_courant returns the value (default 0: the
+                                      entries read it back, synchronising the stream) */
 };
 int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value);
+
+/* No counterpart in the reference (its momentum-energy calls always copy minDtCourant to
+ * the host).  For callers that keep results on the device between calls, with
+ * SPHX_OPT_DEFER_MIN_DT set: the Courant time step of the context's last
+ * sphx_compute_momentum_energy, sphx_compute_hydro_std or sphx_compute_momentum_energy_ve
+ * call, waiting only for that value's copy (INFINITY before any call).  Switching the
+ * option off or the context's stream reads a pending value first. */
+int sphx_min_dt_courant(sphx_ctx* ctx, double* out);
 
 /* statistics of the last call that produced them */
 enum

@@ -22,6 +22,7 @@ OPT_FP64_EXACT = 1
 OPT_TIMING = 2
 OPT_LOOP_SHAPE_FP64 = 3
 OPT_LOOP_SHAPE_MIXED = 4
+OPT_DEFER_MIN_DT = 5
 STAT_CANDIDATE_TESTS = 1
 STAT_SCRATCH_BYTES = 2
 GATHER_MAX_ARRAYS = 16
@@ -106,6 +107,7 @@ SIGNATURES = {
     "sphx_compute_iad": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_momentum_energy": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
     "sphx_compute_hydro_std": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box)]),
+    "sphx_min_dt_courant": (C.c_int, [_V, C.POINTER(_D)]),
     "sphx_density_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V, _V, _V,
                                        _V, _V]),
     "sphx_momentum_energy_adjoint": (C.c_int, [_V, C.POINTER(Groups), C.POINTER(Particles), C.POINTER(Box), _V,

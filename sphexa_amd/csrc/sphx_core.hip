@@ -1042,6 +1042,14 @@ int sphx_ctx_create(int device, void* stream, sphx_ctx** out)
         (void)hipGetLastError();
         ctx->hostStage = nullptr;
     }
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->minDtHost), 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->minDtEvent, hipEventDisableTiming) != hipSuccess)
+    {
+        (void)hipGetLastError(); // no deferred time step on this context
+        if (ctx->minDtHost) (void)hipHostFree(ctx->minDtHost);
+        ctx->minDtHost  = nullptr;
+        ctx->minDtEvent = nullptr;
+    }
     if (ctx->hostStage && hipEventCreateWithFlags(&ctx->readEvent, hipEventDisableTiming) != hipSuccess)
     {
         (void)hipGetLastError();
@@ -1102,6 +1110,8 @@ int sphx_ctx_destroy(sphx_ctx* ctx)
         if (b.p) (void)hipFree(b.p);
     if (ctx->hostStage) (void)hipHostFree(ctx->hostStage);
     if (ctx->readEvent) (void)hipEventDestroy(ctx->readEvent);
+    if (ctx->minDtHost) (void)hipHostFree(ctx->minDtHost);
+    if (ctx->minDtEvent) (void)hipEventDestroy(ctx->minDtEvent);
     delete ctx;
     return SPHX_OK;
 }
@@ -1109,6 +1119,8 @@ int sphx_ctx_destroy(sphx_ctx* ctx)
 int sphx_ctx_set_stream(sphx_ctx* ctx, void* stream)
 {
     if (!ctx) return SPHX_ERR_ARG;
+    double dt; // a deferred time step is read before its stream is left
+    SPHX_TRY(sphx_min_dt_courant(ctx, &dt));
     ctx->stream = static_cast<hipStream_t>(stream);
     return SPHX_OK;
 }

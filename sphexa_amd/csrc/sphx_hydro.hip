@@ -109,6 +109,18 @@ int sphx_ctx_set_option(sphx_ctx* ctx, int option, int value)
         ctx->loopShape[option == SPHX_OPT_LOOP_SHAPE_FP64 ? 0 : 1] = value;
         return SPHX_OK;
     }
+    if (option == SPHX_OPT_DEFER_MIN_DT)
+    {
+        if (value != 0 && value != 1) return argError(ctx, "ctx_set_option: defer_min_dt must be 0 or 1");
+        if (value && !ctx->minDtHost) return fail(ctx, SPHX_ERR_HIP, "ctx_set_option: no pinned memory to defer into");
+        if (!value)
+        {
+            double dt; // a pending time step stays readable
+            SPHX_TRY(sphx_min_dt_courant(ctx, &dt));
+        }
+        ctx->deferMinDt = value;
+        return SPHX_OK;
+    }
     return argError(ctx, "ctx_set_option: unknown option");
 }
 
@@ -240,6 +252,52 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
                             : fast::launchIAD(ctx, G, lists, d, db, false, false);
 }
 
+// The Courant time step of a momentum-energy call: read back (the stream synchronised), or
+// with SPHX_OPT_DEFER_MIN_DT copied to pinned memory behind the loop and d->minDtCourant
+// set to NaN until sphx_min_dt_courant waits for it.  The device minimum is bits of a
+// non-negative double (ordered as integers).
+static int finishMinDt(sphx_ctx* ctx, const unsigned long long* dt, sphx_particles* d)
+{
+    if (ctx->deferMinDt)
+    {
+        SPHX_HIP(ctx, hipMemcpyAsync(ctx->minDtHost, dt, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SPHX_HIP(ctx, hipEventRecord(ctx->minDtEvent, ctx->stream));
+        ctx->minDtPending = true;
+        d->minDtCourant   = NAN;
+        return SPHX_OK;
+    }
+    unsigned long long bits = 0;
+    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    d->minDtCourant   = v;
+    ctx->minDtPending = false;
+    ctx->minDtLast    = v;
+    return SPHX_OK;
+}
+
+// a call without targets: no pairs, no limit
+static int noMinDt(sphx_ctx* ctx, sphx_particles* d)
+{
+    d->minDtCourant   = INFINITY;
+    ctx->minDtPending = false;
+    ctx->minDtLast    = INFINITY;
+    return SPHX_OK;
+}
+
+int sphx_min_dt_courant(sphx_ctx* ctx, double* out)
+{
+    if (!ctx || !out) return SPHX_ERR_ARG;
+    if (ctx->minDtPending)
+    {
+        SPHX_HIP(ctx, hipEventSynchronize(ctx->minDtEvent));
+        std::memcpy(&ctx->minDtLast, ctx->minDtHost, sizeof(double));
+        ctx->minDtPending = false;
+    }
+    *out = ctx->minDtLast;
+    return SPHX_OK;
+}
+
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
 {
     if (!ctx) return SPHX_ERR_ARG;
@@ -252,7 +310,7 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     SphLists        lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     d->minDtCourant = INFINITY;
-    if (G.numGroups == 0) return SPHX_OK;
+    if (G.numGroups == 0) return noMinDt(ctx, d);
     auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
     if (!dt) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
@@ -260,12 +318,7 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
     SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt, false)
                               : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt, false));
-    unsigned long long bits = 0;
-    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
-    double v;
-    std::memcpy(&v, &bits, sizeof(v));
-    d->minDtCourant = v;
-    return SPHX_OK;
+    return finishMinDt(ctx, dt, d);
 }
 
 int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box)
@@ -282,7 +335,7 @@ int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_partic
     SphLists  lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     d->minDtCourant = INFINITY;
-    if (G.numGroups == 0) return SPHX_OK;
+    if (G.numGroups == 0) return noMinDt(ctx, d);
     auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
     if (!dt) return SPHX_ERR_NOMEM;
     // every particle a target: the density loop's epilogue writes the IAD records and, in
@@ -301,12 +354,7 @@ int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_partic
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
     SPHX_TRY(ex ? exact::launchMomentumEnergy(ctx, G, lists, d, db, dt, meOff)
                 : fast::launchMomentumEnergy(ctx, G, lists, d, db, dt, meOff));
-    unsigned long long bits = 0;
-    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
-    double v;
-    std::memcpy(&v, &bits, sizeof(v));
-    d->minDtCourant = v;
-    return SPHX_OK;
+    return finishMinDt(ctx, dt, d);
 }
 
 int sphx_kernel_derivative_tables(double sincIndex, double* whd64, float* whd32)
@@ -453,7 +501,7 @@ int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sp
     SphLists  lists;
     SPHX_TRY(listsFor(ctx, groups, d, G, lists));
     d->minDtCourant = INFINITY;
-    if (G.numGroups == 0) return SPHX_OK;
+    if (G.numGroups == 0) return noMinDt(ctx, d);
     auto* dt = scratchT<unsigned long long>(ctx, S_SMALL, 4);
     if (!dt) return SPHX_ERR_NOMEM;
     DBox db = makeDBox(box);
@@ -461,12 +509,7 @@ int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sp
     SPHX_LAUNCH_CHECK(ctx, "initMinDt");
     SPHX_TRY(useExact(ctx, d) ? exact::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt)
                               : fast::launchMomentumEnergyVe(ctx, G, lists, d, ve, db, dt));
-    unsigned long long bits = 0;
-    SPHX_TRY(readBack(ctx, dt, &bits, sizeof(bits)));
-    double v;
-    std::memcpy(&v, &bits, sizeof(v));
-    d->minDtCourant = v;
-    return SPHX_OK;
+    return finishMinDt(ctx, dt, d);
 }
 
 int sphx_rho_timestep(sphx_ctx* ctx, uint32_t first, uint32_t last, const sphx_particles* d,

@@ -205,6 +205,14 @@ struct sphx_ctx
     // reads) and the event that marks a started read
     void*      hostStage = nullptr;
     hipEvent_t readEvent = nullptr;
+    // deferred Courant time step (SPHX_OPT_DEFER_MIN_DT): the momentum-energy entries copy
+    // the reduced minimum into minDtHost (pinned) and record minDtEvent; sphx_min_dt_courant
+    // waits for it.  minDtLast: the last value read.
+    int        deferMinDt   = 0;
+    double*    minDtHost    = nullptr;
+    hipEvent_t minDtEvent   = nullptr;
+    bool       minDtPending = false;
+    double     minDtLast    = INFINITY;
 };
 
 namespace sphx

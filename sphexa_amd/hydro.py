@@ -43,6 +43,7 @@ from __future__ import annotations
 import contextlib
 import ctypes as C
 import math
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -117,10 +118,45 @@ class Context:
         if st != 0:
             raise SphxError(st, "sphx_ctx_create failed")
         self.h = h
+        self._min_dt_owner = None  # weak reference to the ParticlesData whose time step is pending
+        self.defer_min_dt(True)
 
     def check(self, st):
         if st != 0:
             raise SphxError(st, self.lib.sphx_last_error(self.h).decode())
+
+    def defer_min_dt(self, on: bool):
+        """Momentum-energy calls leave their Courant time step on the device (on, the default
+        here) and ParticlesData.minDtCourant waits for it when read, or each call reads it
+        back at once (off, the library's default)."""
+        self._resolve_min_dt()
+        self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_DEFER_MIN_DT, int(bool(on))))
+
+    def min_dt_courant(self) -> float:
+        """The Courant time step of the context's last momentum-energy call (waits for it)."""
+        v = C.c_double()
+        self.check(self.lib.sphx_min_dt_courant(self.h, C.byref(v)))
+        return v.value
+
+    def _resolve_min_dt(self):
+        """Read a pending time step into its ParticlesData before another call replaces it."""
+        owner = self._min_dt_owner() if self._min_dt_owner is not None else None
+        self._min_dt_owner = None
+        if owner is not None and owner._min_dt_ctx is self:
+            owner.minDtCourant = self.min_dt_courant()
+
+    def _min_dt_call(self, d, call):
+        """Run a momentum-energy call (returning its sphx_particles view) and give d its time
+        step: the value, or the context to wait on when the call deferred it."""
+        owner = self._min_dt_owner() if self._min_dt_owner is not None else None
+        if owner is not None and owner is not d:
+            self._resolve_min_dt()
+        s = call()
+        if math.isnan(s.minDtCourant):
+            d._min_dt, d._min_dt_ctx = math.nan, self
+            self._min_dt_owner = weakref.ref(d)
+        else:
+            d.minDtCourant = s.minDtCourant
 
     def set_fp64_exact(self, on: bool):
         self.check(self.lib.sphx_ctx_set_option(self.h, _lib.OPT_FP64_EXACT, int(bool(on))))
@@ -167,6 +203,7 @@ class Context:
 
     def close(self):
         if self.h:
+            self._resolve_min_dt()  # a pending time step stays readable
             self.lib.sphx_ctx_destroy(self.h)
             self.h = None
 
@@ -404,11 +441,24 @@ class ParticlesData:
     nc: torch.Tensor | None = None
     keys: torch.Tensor | None = None
     wh: torch.Tensor | None = None
-    minDtCourant: float = math.inf
     x_m1: torch.Tensor | None = None
     y_m1: torch.Tensor | None = None
     z_m1: torch.Tensor | None = None
     du_m1: torch.Tensor | None = None
+    # the Courant time step of the last momentum-energy call (minDtCourant), and the context
+    # to read it from while that call's value is still on the device
+    _min_dt: float = field(default=math.inf, init=False, repr=False)
+    _min_dt_ctx: object = field(default=None, init=False, repr=False, compare=False)
+
+    @property
+    def minDtCourant(self) -> float:
+        if self._min_dt_ctx is not None:
+            self._min_dt, self._min_dt_ctx = self._min_dt_ctx.min_dt_courant(), None
+        return self._min_dt
+
+    @minDtCourant.setter
+    def minDtCourant(self, v: float):
+        self._min_dt, self._min_dt_ctx = float(v), None
 
     HYDRO = ("h", "m", "vx", "vy", "vz", "u", "rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33", "ax",
              "ay", "az", "du")
@@ -460,7 +510,7 @@ class ParticlesData:
                     _dev(getattr(self, g), w, m, f"ParticlesData.{g}", optional=True, on_device=False)
                 for g, w, m in self._spec:
                     _dev(getattr(self, g), w, m, f"ParticlesData.{g}", optional=True)
-        s.minDtCourant = self.minDtCourant
+        s.minDtCourant = self._min_dt if self._min_dt_ctx is None else math.nan
         return s
 
 
@@ -538,21 +588,26 @@ def compute_iad(ctx: Context, d: ParticlesData, box: Box, groups=None):
 
 
 def compute_momentum_energy_std(ctx: Context, d: ParticlesData, box: Box, groups=None):
-    s = d.struct()
-    ctx.check(ctx.lib.sphx_compute_momentum_energy(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
-                                                   C.byref(box)))
-    d.minDtCourant = s.minDtCourant
-    return d.minDtCourant
+    """Sets d.minDtCourant (read from the device when first used)."""
+    def call():
+        s = d.struct()
+        ctx.check(ctx.lib.sphx_compute_momentum_energy(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
+                                                       C.byref(box)))
+        return s
+    ctx._min_dt_call(d, call)
 
 
 def compute_hydro_std(ctx: Context, d: ParticlesData, box: Box, groups=None):
     """Density, EOS, IAD and momentum-energy in one call, the order of HydroProp::computeForces
     (main/src/propagator/std_hydro.hpp:139-153); same results as the four calls, with each
-    loop writing the next loop's neighbor records when the groups cover every particle."""
-    s = d.struct()
-    ctx.check(ctx.lib.sphx_compute_hydro_std(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s), C.byref(box)))
-    d.minDtCourant = s.minDtCourant
-    return d.minDtCourant
+    loop writing the next loop's neighbor records when the groups cover every particle.  Sets
+    d.minDtCourant (read from the device when first used)."""
+    def call():
+        s = d.struct()
+        ctx.check(ctx.lib.sphx_compute_hydro_std(ctx.h, C.byref(_groups_struct(groups, d)), C.byref(s),
+                                                 C.byref(box)))
+        return s
+    ctx._min_dt_call(d, call)
 
 
 # ------------------------------------------------------------------------------------------- integration
@@ -713,9 +768,8 @@ def rho_timestep(ctx: Context, d: ParticlesData, ve: VeData, Krho: float = 0.06,
 
 
 def compute_momentum_energy_ve(ctx: Context, d: ParticlesData, ve: VeData, box: Box, groups=None):
-    s = _ve_call(ctx, ctx.lib.sphx_compute_momentum_energy_ve, d, ve, box, groups)
-    d.minDtCourant = s.minDtCourant
-    return d.minDtCourant
+    """Sets d.minDtCourant (read from the device when first used)."""
+    ctx._min_dt_call(d, lambda: _ve_call(ctx, ctx.lib.sphx_compute_momentum_energy_ve, d, ve, box, groups))
 
 
 # ------------------------------------------------------------------------------------------- adjoints

@@ -587,6 +587,51 @@ def test_hydro_std_one_call_equals_four_calls(ctx, mode):
     assert out[True]["dt"] == out[False]["dt"]
 
 
+def test_min_dt_deferred(ctx):
+    """With the time step deferred (the Python default) the momentum-energy calls return
+    without reading minDtCourant: the sphx_particles view holds NaN, sphx_min_dt_courant
+    waits for the value, and ParticlesData.minDtCourant reads it when first used.  Values
+    equal those read back at once; a pending value is read into its ParticlesData before a
+    call on another one replaces it, and when the option is switched off."""
+    import ctypes as C
+    from sphexa_amd import hydro
+    from sphexa_amd._lib import MIXED
+
+    print("case deferred minDtCourant", flush=True)
+    inp, box = uniform_box(20000, seed=65)
+    inp2, _ = uniform_box(20000, seed=66)
+    tin = {k: (dev(v) if k in ("x", "y", "z") else dev(v).float()) for k, v in inp.items()}
+    tin2 = {k: (dev(v) if k in ("x", "y", "z") else dev(v).float()) for k, v in inp2.items()}
+    want = {}
+    try:
+        ctx.defer_min_dt(False)
+        for name, t in (("a", tin), ("b", tin2)):
+            step = hydro.HydroStep(ctx, len(t["x"]), hbox(box), MIXED)
+            d = step.compute_forces(t)
+            assert d._min_dt_ctx is None
+            want[name] = d.minDtCourant
+        assert np.isfinite(want["a"]) and want["a"] > 0 and want["a"] != want["b"]
+
+        ctx.defer_min_dt(True)
+        sa = hydro.HydroStep(ctx, len(tin["x"]), hbox(box), MIXED)
+        sb = hydro.HydroStep(ctx, len(tin2["x"]), hbox(box), MIXED)
+        da = sa.compute_forces(tin)
+        assert da._min_dt_ctx is ctx  # pending
+        assert np.isnan(da.struct().minDtCourant)
+        db = sb.compute_forces(tin2)  # reads da's value first
+        assert da._min_dt_ctx is None and da.minDtCourant == want["a"]
+        assert db._min_dt_ctx is ctx
+        v = C.c_double()
+        assert ctx.lib.sphx_min_dt_courant(ctx.h, C.byref(v)) == 0 and v.value == want["b"]
+        assert db.minDtCourant == want["b"] and db._min_dt_ctx is None
+        for _ in range(2):  # repeated calls on one ParticlesData: the latest value
+            da = sa.compute_forces(tin)
+        ctx.defer_min_dt(False)  # reads the pending value
+        assert da._min_dt_ctx is None and da.minDtCourant == want["a"]
+    finally:
+        ctx.defer_min_dt(True)
+
+
 def test_hydro_std_one_call_sub_range(ctx):
     """Groups over a sub-range [first, last) (the particles outside are neighbors only): the
     one call packs the records as the four calls do and gives their bits on [first, last)."""

@@ -61,12 +61,14 @@ __global__ __launch_bounds__(256) void csCountSplits(const uint64_t* __restrict_
     }
 }
 
+// sOfs: the exclusive scan of the split counts, n + 1 entries; a particle that starts no
+// split node (sOfs[i + 1] == sOfs[i], nearly all of them) reads no key
 __global__ __launch_bounds__(256) void csEmitChildren(const uint64_t* __restrict__ keys, uint32_t n, uint32_t bucket,
                                                       const uint32_t* __restrict__ sOfs,
                                                       uint64_t* __restrict__ childKeys)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || sOfs[i + 1] == sOfs[i]) return;
     int lo, hi;
     splitLevels(keys, n, bucket, i, lo, hi);
     uint64_t k    = keys[i];

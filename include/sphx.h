@@ -311,7 +311,7 @@ int sphx_compute_iad(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d
 /* sph::computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:41-101,
  * momentumAndEnergyJLoop hydro_std/momentum_energy_kern.hpp:12-134): ax, ay, az,
  * du and d->minDtCourant (Courant time step, kernels.hpp:10-17).  Synchronises
- * the stream once to return minDtCourant. */
+ * the stream once to return minDtCourant (with SPHX_OPT_DEFER_MIN_DT: not at all). */
 int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                  const sphx_box* box);
 
@@ -321,7 +321,8 @@ int sphx_compute_momentum_energy(sphx_ctx* ctx, const sphx_groups* groups, sphx_
  * them).  Same results as the four calls.  When the groups cover every particle, the
  * density loop's epilogue writes the IAD loop's neighbor records (x, y, z, m/rho) and,
  * for MIXED fields, the IAD loop's epilogue the momentum-energy records, in place of
- * packing launches.  Synchronises the stream once to return minDtCourant. */
+ * packing launches.  Synchronises the stream once to return minDtCourant (with
+ * SPHX_OPT_DEFER_MIN_DT: not at all). */
 int sphx_compute_hydro_std(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d, const sphx_box* box);
 
 /* ---- integration step ------------------------------------------------------ */
@@ -423,7 +424,8 @@ int sphx_compute_av_switches(sphx_ctx* ctx, const sphx_groups* groups, sphx_part
 /* sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:40-117,
  * momentumAndEnergyJLoop momentum_energy_kern.hpp:65-221, tdpdTrho absent), avClean =
  * ve->avClean (then the AV correction avRvCorrection :43-63 reads dV11..dV33): ax, ay, az,
- * du and d->minDtCourant.  Synchronises the stream once to return minDtCourant. */
+ * du and d->minDtCourant.  Synchronises the stream once to return minDtCourant (with
+ * SPHX_OPT_DEFER_MIN_DT: not at all). */
 int sphx_compute_momentum_energy_ve(sphx_ctx* ctx, const sphx_groups* groups, sphx_particles* d,
                                     sphx_ve_fields* ve, const sphx_box* box);
 

@@ -472,7 +472,7 @@ static int radixPass(sphx_ctx* ctx, sphx_ctx* tctx, const uint64_t* kSrc, const 
 // throughout (no host synchronisation, no memset, no timing events).  Both give the same
 // result: a pass over a constant digit is the identity permutation.
 static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits,
-                         const SortWork* own, int lowBits = 0)
+                         const SortWork* own, int lowBits = 0, bool checkDigits = true)
 {
     if (n <= 1) return SPHX_OK;
     if (n >= (uint64_t(1) << 32)) return argError(ctx, "sort: n must be < 2^32");
@@ -493,7 +493,7 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
         if (!kAlt || (vals && !vAlt) || !hist) return SPHX_ERR_NOMEM;
     }
     uint64_t *tileVar = nullptr, *vary = nullptr;
-    if (!own && keyBits - 8 * (lowBits / 8) > 8)
+    if (!own && checkDigits && keyBits - 8 * (lowBits / 8) > 8)
     {
         tileVar = scratchT<uint64_t>(ctx, S_SORT_VAR, 2 * uint64_t(numTiles));
         vary    = scratchT<uint64_t>(ctx, S_SMALL, 8);
@@ -537,9 +537,10 @@ static int radixSortWith(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t
     return SPHX_OK;
 }
 
-int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits)
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits,
+                   bool checkDigits)
 {
-    return radixSortWith(ctx, keys, vals, n, keyBits, nullptr, std::max(0, std::min(lowBits, keyBits)));
+    return radixSortWith(ctx, keys, vals, n, keyBits, nullptr, std::max(0, std::min(lowBits, keyBits)), checkDigits);
 }
 
 // ---------------------------------------------------------------------------

@@ -307,7 +307,9 @@ int exclusiveScanU32(sphx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t 
 // only the bits in [0, keyBits) are examined
 // lowBits: bits below it are known to be equal across all keys (passes start at the digit
 // holding bit lowBits)
-int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits = 0);
+// checkDigits = false: no constant-digit check (no host read; every pass from lowBits runs)
+int radixSortPairs(sphx_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, int keyBits, int lowBits = 0,
+                   bool checkDigits = true);
 
 // ---------------------------------------------------------------------------
 // key bit algebra (64-bit Hilbert keys, 21 levels, 63 key bits)

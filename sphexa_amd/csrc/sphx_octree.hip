@@ -286,8 +286,12 @@ int sphx_compute_octree(sphx_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
         }
         SPHX_LAUNCH_CHECK(ctx, "csEmitChildren");
         // children of level-l nodes are multiples of the level-(l + 1) range 8^(20 - l): the
-        // low 3 (20 - maxSplit) bits of every child key are zero
-        SPHX_TRY(radixSortPairs(ctx, child, nullptr, L, 64, 3 * (int(kMaxLevel) - 1 - int(maxSplit))));
+        // low 3 (20 - maxSplit) bits of every child key are zero.  The digits above all but
+        // always vary (the root's children differ in bits 60..62, level maxSplit + 1
+        // children in the lowest digit kept), so the sort runs them without the
+        // constant-digit check and its host read (a pass over a constant digit changes
+        // nothing)
+        SPHX_TRY(radixSortPairs(ctx, child, nullptr, L, 64, 3 * (int(kMaxLevel) - 1 - int(maxSplit)), false));
     }
     csFinishLeaves<<<1, 64, 0, ctx->stream>>>(child, uint32_t(L));
     SPHX_LAUNCH_CHECK(ctx, "csFinishLeaves");

@@ -727,6 +727,9 @@ __global__ __launch_bounds__(256) void grpSplitMasks(uint32_t first, uint32_t la
     const uint32_t p0 = first + 64u * g;
     const uint32_t p1 = min(p0 + 63u, last - 1);
     const uint32_t bi = min(p0 + lane, p1);
+    // the particle's fields load while the leaf search below runs (they do not depend on it)
+    const double xb = x[bi], yb = y[bi], zb = z[bi];
+    const double hb = h ? double(h[bi]) : 0.0;
 
     int lo = 0, hi = numLeaves; // layout[lo] <= p0; leaf(p0) in [lo, hi)
     while (hi - lo > 1)
@@ -750,8 +753,8 @@ __global__ __launch_bounds__(256) void grpSplitMasks(uint32_t first, uint32_t la
     int      maxLevel = int(waveMax(int(level)));
     double   distCrit = ldexp(1.0, -maxLevel) * double(tolFactor);
     double   minExt   = fmin(fmin(box.len[0], box.len[1]), box.len[2]);
-    double   px = x[bi] * box.ilen[0], py = y[bi] * box.ilen[1], pz = z[bi] * box.ilen[2];
-    double   pw = h ? 2.0 * double(h[bi]) / minExt : 1.0;
+    double   px = xb * box.ilen[0], py = yb * box.ilen[1], pz = zb * box.ilen[2];
+    double   pw = h ? 2.0 * hb / minExt : 1.0;
     double   qx = __shfl_down(px, 1, 64), qy = __shfl_down(py, 1, 64), qz = __shfl_down(pz, 1, 64);
     if (lane == 63) qx = px, qy = py, qz = pz;
     double   dx = qx - px, dy = qy - py, dz = qz - pz;

@@ -93,3 +93,58 @@ def test_limit_box_shrinking_matches_oracle_and_reference():
         mine = _as_tuple(hydro.limit_box_shrinking(fit, prev))
         for lib in libs:
             assert mine == _as_tuple(lib.limit_box_shrinking(fit, prev)), lib.pre
+
+
+def test_deferred_min_dt_bookkeeping():
+    """The Python side of the deferred Courant time step, with a stand-in for the library
+    (no GPU): a call whose sphx_particles view comes back with NaN leaves its ParticlesData
+    pending on the context; reading minDtCourant asks the context once; a call for another
+    ParticlesData reads the pending one's value first (the library keeps only the last);
+    a call that read the value back sets it at once; assigning clears the pending state;
+    the view a call receives holds NaN while a value is pending."""
+    library = {"value": None}  # the value sphx_min_dt_courant would return
+    asked = []
+
+    ctx = object.__new__(hydro.Context)  # no device: only the bookkeeping is exercised
+    ctx.h = None
+    ctx._min_dt_owner = None
+
+    def min_dt_courant():
+        asked.append(library["value"])
+        return library["value"]
+
+    ctx.min_dt_courant = min_dt_courant
+
+    def deferred_call(d, v):
+        def call():
+            s = d.struct()
+            library["value"] = v
+            s.minDtCourant = float("nan")
+            return s
+        return call
+
+    def immediate_call(d, v):
+        def call():
+            s = d.struct()
+            s.minDtCourant = v
+            return s
+        return call
+
+    d1, d2 = hydro.ParticlesData(n=4), hydro.ParticlesData(n=4)
+    assert d1.minDtCourant == float("inf")
+    ctx._min_dt_call(d1, deferred_call(d1, 0.25))
+    assert d1._min_dt_ctx is ctx and asked == []
+    assert np.isnan(d1.struct().minDtCourant)
+    ctx._min_dt_call(d1, deferred_call(d1, 0.5))  # the same ParticlesData: the newer value wins
+    assert asked == []
+    ctx._min_dt_call(d2, deferred_call(d2, 0.125))  # d1 is read before d2's call replaces it
+    assert asked == [0.5] and d1._min_dt_ctx is None and d1.minDtCourant == 0.5
+    assert d2.minDtCourant == 0.125 and d2.minDtCourant == 0.125 and asked == [0.5, 0.125]
+    ctx._min_dt_call(d1, immediate_call(d1, 0.75))
+    assert d1.minDtCourant == 0.75 and d1._min_dt_ctx is None and d1.struct().minDtCourant == 0.75
+    ctx._min_dt_call(d2, deferred_call(d2, 1.5))
+    d2.minDtCourant = 3.0  # an assignment replaces the pending value
+    assert d2.minDtCourant == 3.0 and asked == [0.5, 0.125]
+    ctx._min_dt_call(d1, deferred_call(d1, 2.0))
+    ctx._resolve_min_dt()  # what switching the option off and closing do first
+    assert d1._min_dt_ctx is None and d1.minDtCourant == 2.0

@@ -860,7 +860,7 @@ static int runSearch(sphx_ctx* ctx, const GroupsDev& G, const double* x, const d
             return fail(ctx, SPHX_ERR_STACK,
                         "neighbor search: a neighbor count to update h from exceeds the factor table (" +
                             std::to_string(kHFactorCap) + ")");
-        if (!Sph) break;
+        if (!Sph || hf.redoCount == 0) break;
         numLaunch = hf.redoCount;
         list      = outList;
         pingpong ^= 1;

@@ -212,7 +212,7 @@ int sphx_sort_by_key(sphx_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t n
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
  * stable sort as sphx_sort_by_key, bit for bit, with every work buffer in one fixed device
- * array of the library (about 24.7 MiB, one copy per GPU, held until the process ends).
+ * array of the library (about 25.3 MiB, one copy per GPU, held until the process ends).
  * n must be <= 2097152 (2^21) keys, else SPHX_ERR_ARG.  It allocates nothing, never
  * synchronises the stream and sets no memory with memset (every digit pass runs).  All
  * calls on one GPU share the array: calls that overlap on one GPU race. */
